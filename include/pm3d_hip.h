@@ -1,0 +1,125 @@
+/* C-ABI of the MI355X (gfx950) particle-mesh library `pbh_amd/_pm3d_hip.so`.
+ *
+ * It replaces the numpy hot path of the reference module pbhgr/pbhgr/pm3d.py (uniquehuman/pbh):
+ * CIC deposit, FFT Poisson solve with the spectral gradient, CIC gather and the KDK leapfrog in a.
+ * The calling convention follows the reference's own native boundary (pbhgr/csrc/kernels.c with
+ * its ctypes loader pbhgr/pbhgr/cosmo_ev.py:31-47): plain C functions, `int` return, scalars as
+ * int/double, arrays as caller-owned host pointers plus explicit sizes.  One deviation: device state
+ * (positions, momenta, meshes) lives behind an opaque handle so that it stays resident in HBM.
+ *
+ * Return values: 0 = OK; > 0 = count of flagged particles (non-finite position or momentum, the
+ *   analogue of the reference kernels' "bad cell" counts, kernels.c:36,73,105,166);
+ *   < 0 = error (see PM3D_E*), with a message from pm3d_last_error().
+ * Host arrays are C-contiguous.  Particle arrays are (n,3) FP64 in Lagrangian order (particle i is
+ * lattice site i in C order, pm3d.py:64-65).  Meshes are (N,N,N) C order.  Half spectra are
+ * (N,N,N/2+1) complex128 stored as interleaved (re,im) doubles, numpy's rfftn layout.
+ * A handle is bound to one HIP stream and is not thread-safe; distinct handles may be used from
+ * distinct host threads.
+ */
+#ifndef PM3D_HIP_H
+#define PM3D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM3D_FP64 0   /* FP64 positions/momenta/FFT; int64 fixed-point deposit */
+#define PM3D_MIXED 1  /* int32 cell + FP32 offset, FP32 momenta/FFT/forces; int64 deposit; FP64 scalars */
+
+#define PM3D_OK 0
+#define PM3D_EINVAL (-1)
+#define PM3D_EHIP (-2)
+#define PM3D_ENOMEM (-3)
+#define PM3D_EUNSUPPORTED (-4)
+#define PM3D_ESTATE (-5)
+
+typedef struct pm3d_cfg {
+    int32_t N;          /* mesh cells per side: power of two in [8, 512] */
+    int32_t Np;         /* particles per side (n = Np^3) */
+    double L;           /* periodic box side */
+    double smooth;      /* Gaussian force smoothing length in cells, 0 = none (pm3d.py:27,107-108) */
+    int32_t precision;  /* PM3D_FP64 or PM3D_MIXED */
+    int32_t frac_bits;  /* fixed-point fraction bits of the deposit; 0 selects 40 */
+} pm3d_cfg;
+
+/* Host scalars of one KDK step, evaluated in Python exactly as pm3d.py:118-127 does
+ * (`**` is libm pow, so the device never recomputes them). */
+typedef struct pm3d_step_scalars {
+    double a;          /* a at the start of the step */
+    double da;
+    double c1;         /* 0.5*da */
+    double den1;       /* a*H(a) */
+    double den_d;      /* a_mid**3 * H(a_mid) */
+    double den2;       /* a_new*H(a_new) */
+    double s_old;      /* -1.5*H_I**2/a     (used only when the cached acceleration is stale) */
+    double s_new;      /* -1.5*H_I**2/a_new */
+    double sm2;        /* (smooth*dx)**2 */
+    int32_t need_forces; /* 1: recompute the opening acceleration at a (pm3d.py:119-120) */
+    int32_t pad;
+} pm3d_step_scalars;
+
+/* Lifecycle.  Replaces PM3D.__init__ (pm3d.py:27-37): the k-grids are regenerated on the device
+ * from integer indices with numpy's fftfreq formula instead of being stored. */
+int pm3d_create(const pm3d_cfg* cfg, void** handle);
+int pm3d_destroy(void* handle);
+const char* pm3d_last_error(void* handle);  /* handle may be NULL: the last create error */
+int pm3d_synchronize(void* handle);         /* waits for the stream; returns the sticky bad count */
+int64_t pm3d_device_bytes(void* handle);    /* HBM held by the handle */
+
+/* State transfer (the reference keeps pm.x / pm.p as numpy attributes, pm3d.py:67-68,124). */
+int pm3d_set_state(void* handle, const double* x, const double* p, int64_t n);
+int pm3d_get_state(void* handle, double* x, double* p);  /* either pointer may be NULL */
+
+/* PM3D._cic_index on the three axes (pm3d.py:76-79): key[i] = (ix0*N+iy0)*N+iz0 (int64, the
+ * deposit's flat index of corner 0, pm3d.py:88) and frac[i,d] = g - floor(g). */
+int pm3d_cic_index(void* handle, const double* X, int64_t n, int64_t* key, double* frac);
+
+/* PM3D.deposit(X) (pm3d.py:81-89): particle counts per cell, (N,N,N) FP64, from X (n,3). */
+int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho);
+
+/* PM3D.interp(F, X) (pm3d.py:91-99): CIC gather of the (N,N,N) mesh F at X (n,3) into out (n,). */
+int pm3d_interp(void* handle, const double* F, const double* X, int64_t n, double* out);
+
+/* PM3D.forces(a) (pm3d.py:101-114) at the current state: acc (n,3) and delta (N,N,N); either
+ * output may be NULL.  s = -1.5*H_I**2/a and sm2 = (smooth*dx)**2 come from the host.  Like the
+ * reference, it leaves the step's cached opening acceleration untouched. */
+int pm3d_forces(void* handle, double a, double s, double sm2, double* acc, double* delta);
+
+/* PM3D.step(a, da) (pm3d.py:117-129).  delta (N,N,N) and delta_max may be NULL; when both are
+ * NULL the call is asynchronous (no host synchronisation). */
+int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* delta_max);
+
+/* Lazy access to the density contrast of a deposit (the `delta` that PM3D.step returns,
+ * pm3d.py:104,129).  Every deposit of the state gets a serial number; the maximum of delta of the
+ * last 4096 deposits stays on the device.  pm3d_get_delta converts the most recent deposit. */
+int64_t pm3d_deposit_serial(void* handle);
+int pm3d_delta_max_at(void* handle, int64_t serial, double* delta_max);
+int pm3d_get_delta(void* handle, double* delta);
+
+/* numpy.fft.rfftn / irfftn(s=(N,N,N)) on the device FFT (pm3d.py:105,112; numpy conventions:
+ * unnormalised forward, 1/N^3 inverse, c2r ignores the imaginary parts of the kz=0 and kz=N/2 bins). */
+int pm3d_rfftn(void* handle, const double* in, double* out_c);
+int pm3d_irfftn(void* handle, const double* in_c, double* out);
+
+/* Spectral gradient meshes g_c = irfftn(-1j*k_c*phik) of pm3d.py:105-112 for a given real mesh
+ * delta: g (N,N,N,3) interleaved.  Used by the parity tests of the fused Poisson pass. */
+int pm3d_poisson_gradient(void* handle, const double* delta, double s, double sm2, double* g);
+
+/* First-shell-crossing diagnostic (not in the reference; defined by this library):
+ * min over Lagrangian lattice cells of det(dx/dq) with forward differences along the q-lattice,
+ * minimum-image unwrapped, in units of the lattice spacing.  Requires Np == particles per side. */
+int pm3d_min_jacobian(void* handle, double* jmin);
+
+/* Per-kernel timing with hipEvents recorded on the handle's stream.  When enabled, every launch
+ * of the step is bracketed by events; pm3d_get_timings returns (count, total ms) per kernel class
+ * in the order of pm3d_kernel_names(), and resets the accumulators. */
+int pm3d_set_timing(void* handle, int enable);
+int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax);
+const char* pm3d_kernel_names(void);  /* comma-separated */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PM3D_HIP_H */

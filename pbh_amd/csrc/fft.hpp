@@ -1,0 +1,322 @@
+// Hand-written Stockham FFT for the periodic Poisson solve (gfx950).
+//
+// The 3D real-to-complex transform of an N^3 mesh is done in three pencil passes, each one a
+// kernel that stages P pencils in LDS, runs a radix-4 (+ one radix-2) Stockham FFT there and
+// writes back coalesced rows:
+//   z pass  (contiguous rows): real N -> N/2 complex FFT + even/odd split  -> half spectrum
+//   y pass  (stride N/2)     : complex N, in place
+//   x pass  (stride N*N/2)   : complex N forward, then the Green's function, the optional Gaussian
+//                              filter and -i k_c fused in registers, then three inverse x FFTs
+// and the inverse runs y and z passes back for the three gradient components.
+//
+// Half-spectrum layout: `spec` holds kz = 0..N/2-1 as [kx][ky][kz] (power-of-two rows) and the
+// kz = N/2 Nyquist plane lives in a separate [kx][ky] array `nyq`.  numpy's layout (N,N,N/2+1) is
+// assembled only on the host side of the test entry points.
+//
+// LDS layout of a pencil block: element j of pencil c at buf[j*(P+1) + c]; threads map pencil-fastest
+// in the butterflies so every LDS access of a wave is a contiguous run.
+#pragma once
+#include "common.hpp"
+
+namespace pm3d {
+
+// Pencils per workgroup for each pass: every LDS ping-pong buffer holds about 16 KB.
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+template <typename R, int N> struct Plan {
+    static constexpr int EB = 16384 / (2 * (int)sizeof(R));  // complex elements per buffer
+    static constexpr int PZ = cmin(2 * EB / N, N * N);       // r2c rows (N/2-point FFTs)
+    static constexpr int PY = cmin(EB / N, N / 2);           // y columns
+    static constexpr int PX = cmin(EB / N, N / 2);           // x columns (fused Poisson pass)
+    static constexpr int PC = cmin(EB / N, N * N);           // c2r rows (3 components each)
+};
+
+// In-LDS Stockham FFT of P pencils of NC points (unnormalised; forward = exp(-2 pi i jk/NC)).
+// Input in `a`, result in the returned buffer.  Caller synchronises before the call; the call
+// ends with a barrier.  tw[k] = exp(-2 pi i k / (NC*TWS)).
+template <typename R, int NC, int P, bool INV, int TWS>
+__device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R>* __restrict__ tw) {
+    constexpr int PP = P + 1;
+    cplx<R>* src = a;
+    cplx<R>* dst = b;
+    int Ns = 1;
+#pragma unroll
+    for (; Ns * 4 <= NC; Ns *= 4) {
+        const int s = TWS * (NC / (4 * Ns));
+        for (int t = threadIdx.x; t < P * (NC / 4); t += kThreads) {
+            const int c = t % P, j = t / P;
+            const int k = j & (Ns - 1);
+            cplx<R> v0 = src[j * PP + c];
+            cplx<R> v1 = src[(j + NC / 4) * PP + c];
+            cplx<R> v2 = src[(j + NC / 2) * PP + c];
+            cplx<R> v3 = src[(j + 3 * NC / 4) * PP + c];
+            if (Ns > 1) {
+                cplx<R> w1 = tw[k * s], w2 = tw[2 * k * s], w3 = tw[3 * k * s];
+                if (INV) { w1 = cconj(w1); w2 = cconj(w2); w3 = cconj(w3); }
+                v1 = cmul(v1, w1);
+                v2 = cmul(v2, w2);
+                v3 = cmul(v3, w3);
+            }
+            const cplx<R> a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+            cplx<R> a3 = csub(v1, v3);
+            a3 = INV ? cplx<R>{-a3.im, a3.re} : cplx<R>{a3.im, -a3.re};
+            const int d = (j / Ns) * Ns * 4 + k;
+            dst[d * PP + c] = cadd(a0, a2);
+            dst[(d + Ns) * PP + c] = cadd(a1, a3);
+            dst[(d + 2 * Ns) * PP + c] = csub(a0, a2);
+            dst[(d + 3 * Ns) * PP + c] = csub(a1, a3);
+        }
+        __syncthreads();
+        cplx<R>* tmp = src; src = dst; dst = tmp;
+    }
+    if (Ns < NC) {  // one radix-2 pass when log2(NC) is odd
+        const int s = TWS * (NC / (2 * Ns));
+        for (int t = threadIdx.x; t < P * (NC / 2); t += kThreads) {
+            const int c = t % P, j = t / P;
+            const int k = j & (Ns - 1);
+            cplx<R> v0 = src[j * PP + c];
+            cplx<R> v1 = src[(j + NC / 2) * PP + c];
+            if (Ns > 1) {
+                cplx<R> w = tw[k * s];
+                if (INV) w = cconj(w);
+                v1 = cmul(v1, w);
+            }
+            const int d = (j / Ns) * Ns * 2 + k;
+            dst[d * PP + c] = cadd(v0, v1);
+            dst[(d + Ns) * PP + c] = csub(v0, v1);
+        }
+        __syncthreads();
+        src = dst;
+    }
+    return src;
+}
+
+// ------------------------------------------------------------------------------------------ z r2c
+// Rows (x,y) of the real mesh -> half spectrum along kz.  FIXED: the input is the int64 fixed-point
+// deposit, converted on the fly to delta = rho/n_mean - 1 (pm3d.py:103-104), with the row maxima
+// folded into *qmax and delta optionally written out.
+template <typename R, int N, bool FIXED>
+__global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in, cplx<R>* __restrict__ spec,
+                                                    cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
+                                                    R* __restrict__ delta_out, unsigned long long* __restrict__ qmax,
+                                                    double qscale, double nmean) {
+    constexpr int NC = N / 2, P = Plan<R, N>::PZ, PP = P + 1;
+    __shared__ cplx<R> A[NC * PP], B[NC * PP];
+    const size_t row0 = (size_t)blockIdx.x * P;
+    unsigned long long lmax = 0;
+    for (int t = threadIdx.x; t < P * NC; t += kThreads) {
+        const int c = t / NC, m = t - c * NC;
+        const size_t g = (row0 + c) * N + 2 * m;
+        R v0, v1;
+        if constexpr (FIXED) {
+            const long long q0 = ((const long long*)in)[g], q1 = ((const long long*)in)[g + 1];
+            const unsigned long long u0 = q0 > 0 ? (unsigned long long)q0 : 0ull;
+            const unsigned long long u1 = q1 > 0 ? (unsigned long long)q1 : 0ull;
+            lmax = u0 > lmax ? u0 : lmax;
+            lmax = u1 > lmax ? u1 : lmax;
+            v0 = (R)(((double)q0 * qscale) / nmean - 1.0);
+            v1 = (R)(((double)q1 * qscale) / nmean - 1.0);
+            if (delta_out) {
+                delta_out[g] = v0;
+                delta_out[g + 1] = v1;
+            }
+        } else {
+            v0 = ((const R*)in)[g];
+            v1 = ((const R*)in)[g + 1];
+        }
+        A[m * PP + c] = cplx<R>{v0, v1};
+    }
+    if constexpr (FIXED) {
+        lmax = wave_max_u64(lmax);
+        if ((threadIdx.x & 63) == 0 && lmax) atomicMax(qmax, lmax);
+    }
+    __syncthreads();
+    const cplx<R>* Z = lds_fft<R, NC, P, false, 2>(A, B, tw);
+    for (int t = threadIdx.x; t < P * NC; t += kThreads) {
+        const int c = t / NC, k = t - c * NC;
+        const cplx<R> zk = Z[k * PP + c];
+        const cplx<R> zm = Z[((NC - k) & (NC - 1)) * PP + c];
+        // even part E = (Z_k + conj Z_{-k})/2, odd part O = (Z_k - conj Z_{-k})/(2i)
+        const R er = (zk.re + zm.re) * (R)0.5, ei = (zk.im - zm.im) * (R)0.5;
+        const R orr = (zk.im + zm.im) * (R)0.5, oi = (zm.re - zk.re) * (R)0.5;
+        const cplx<R> w = tw[k];
+        spec[(row0 + c) * NC + k] = cplx<R>{er + (w.re * orr - w.im * oi), ei + (w.re * oi + w.im * orr)};
+        if (k == 0) nyq[row0 + c] = cplx<R>{zk.re - zk.im, (R)0};
+    }
+}
+
+// ------------------------------------------------------------------------------------------ y c2c
+// In-place complex FFT along y on `ncomp` = gridDim.y half spectra (component strides cs, cn).
+template <typename R, int N, bool INV>
+__global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
+                                                    size_t cs, size_t cn, const cplx<R>* __restrict__ tw) {
+    constexpr int P = Plan<R, N>::PY, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
+    __shared__ cplx<R> A[N * PP], B[N * PP];
+    spec += blockIdx.y * cs;
+    nyq += blockIdx.y * cn;
+    const int b = blockIdx.x;
+    if (b < NBM) {
+        const int x = b / (NH / P), col0 = (b % (NH / P)) * P;
+        cplx<R>* base = spec + (size_t)x * N * NH + col0;
+        for (int t = threadIdx.x; t < N * P; t += kThreads) {
+            const int c = t % P, j = t / P;
+            A[j * PP + c] = base[(size_t)j * NH + c];
+        }
+        __syncthreads();
+        const cplx<R>* res = lds_fft<R, N, P, INV, 1>(A, B, tw);
+        for (int t = threadIdx.x; t < N * P; t += kThreads) {
+            const int c = t % P, j = t / P;
+            base[(size_t)j * NH + c] = res[j * PP + c];
+        }
+    } else {
+        cplx<R>* base = nyq + (size_t)(b - NBM) * P * N;  // pencil c = row x0+c, contiguous in y
+        for (int t = threadIdx.x; t < N * P; t += kThreads) {
+            const int j = t % N, c = t / N;
+            A[j * PP + c] = base[(size_t)c * N + j];
+        }
+        __syncthreads();
+        const cplx<R>* res = lds_fft<R, N, P, INV, 1>(A, B, tw);
+        for (int t = threadIdx.x; t < N * P; t += kThreads) {
+            const int j = t % N, c = t / N;
+            base[(size_t)c * N + j] = res[j * PP + c];
+        }
+    }
+}
+
+// numpy fftfreq / rfftfreq: k = (2*pi) * (i * (1/(N*dx))) with the Nyquist index of the full axis
+// negative (pm3d.py:31-32; numpy/fft/_helper.py fftfreq, rfftfreq).
+__device__ __forceinline__ double kfull(int i, int N, double kval) {
+    return 6.283185307179586 * ((double)(i < N / 2 ? i : i - N) * kval);
+}
+__device__ __forceinline__ double khalf(int i, double kval) { return 6.283185307179586 * ((double)i * kval); }
+
+// ------------------------------------------------------------------------------------------ x pass
+// MODE 0: forward FFT along x in place.  MODE 1: inverse in place.  MODE 2 (the Poisson pass):
+// forward FFT along x, then per mode  phik = ((s*dk) * (1/k2)) [* exp((-0.5*k2)*sm2)], phik(0)=0
+// (pm3d.py:105-109; numpy's complex/real division is a multiply by the reciprocal), then for
+// c = x,y,z: -1j*k_c*phik = (k_c*Im, -k_c*Re) (pm3d.py:112) and the inverse FFT along x, written to
+// the three gradient spectra.
+template <typename R, int N, int MODE>
+__global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
+                                                     cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
+                                                     size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
+                                                     double kval, double s, double sm2, int smooth_on) {
+    constexpr int P = Plan<R, N>::PX, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
+    constexpr int EPT = (N * P + kThreads - 1) / kThreads;
+    __shared__ cplx<R> A[N * PP], B[N * PP];
+    const int b = blockIdx.x;
+    const bool main = b < NBM;
+    int y = 0, col0 = 0;
+    if (main) {
+        y = b / (NH / P);
+        col0 = (b % (NH / P)) * P;
+    } else {
+        col0 = (b - NBM) * P;  // first ky of this Nyquist block
+    }
+    // element (x = j, pencil c) offset within a spectrum
+    auto off = [&](int j, int c) -> size_t {
+        return main ? ((size_t)j * N + y) * NH + col0 + c : (size_t)j * N + col0 + c;
+    };
+    cplx<R>* src = main ? spec : nyq;
+    for (int t = threadIdx.x; t < N * P; t += kThreads) {
+        const int c = t % P, j = t / P;
+        A[j * PP + c] = src[off(j, c)];
+    }
+    __syncthreads();
+    if constexpr (MODE < 2) {
+        const cplx<R>* res = lds_fft<R, N, P, MODE == 1, 1>(A, B, tw);
+        for (int t = threadIdx.x; t < N * P; t += kThreads) {
+            const int c = t % P, j = t / P;
+            src[off(j, c)] = res[j * PP + c];
+        }
+        return;
+    } else {
+        const cplx<R>* res = lds_fft<R, N, P, false, 1>(A, B, tw);
+        cplx<R> ph[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int t = threadIdx.x + e * kThreads;
+            ph[e] = cplx<R>{(R)0, (R)0};
+            if (t < N * P) {
+                const int c = t % P, j = t / P;
+                const int iy = main ? y : col0 + c, iz = main ? col0 + c : NH;
+                const double kx = kfull(j, N, kval), ky = kfull(iy, N, kval), kz = khalf(iz, kval);
+                const bool dc = (j == 0) && (iy == 0) && (iz == 0);
+                double k2 = (kx * kx + ky * ky) + kz * kz;
+                if (dc) k2 = 1.0;
+                const double rk = 1.0 / k2;
+                const cplx<R> z = res[j * PP + c];
+                double pr = (s * (double)z.re) * rk, pi = (s * (double)z.im) * rk;
+                if (smooth_on) {
+                    const double f = exp((-0.5 * k2) * sm2);
+                    pr = pr * f;
+                    pi = pi * f;
+                }
+                if (dc) { pr = 0.0; pi = 0.0; }
+                ph[e] = cplx<R>{(R)pr, (R)pi};
+            }
+        }
+        __syncthreads();
+        for (int comp = 0; comp < 3; ++comp) {
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                const int t = threadIdx.x + e * kThreads;
+                if (t < N * P) {
+                    const int c = t % P, j = t / P;
+                    const int iy = main ? y : col0 + c, iz = main ? col0 + c : NH;
+                    const double kc = comp == 0 ? kfull(j, N, kval) : comp == 1 ? kfull(iy, N, kval) : khalf(iz, kval);
+                    A[j * PP + c] = cplx<R>{(R)(kc * (double)ph[e].im), (R)(-(kc * (double)ph[e].re))};
+                }
+            }
+            __syncthreads();
+            const cplx<R>* r2 = lds_fft<R, N, P, true, 1>(A, B, tw);
+            cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
+            for (int t = threadIdx.x; t < N * P; t += kThreads) {
+                const int c = t % P, j = t / P;
+                dst[off(j, c)] = r2[j * PP + c];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------ z c2r
+// Half spectra along kz -> real rows, NCOMP components written interleaved out[cell*NCOMP + comp],
+// scaled by `scale` (1/N^3, numpy's backward normalisation).  numpy's c2r uses only the real parts
+// of the kz = 0 and kz = N/2 bins (numpy/fft/_pocketfft.py irfft), and so does this kernel.
+template <typename R, int N, int NCOMP>
+__global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ gspec, const cplx<R>* __restrict__ gnyq,
+                                                    size_t gcs, size_t gcn, R* __restrict__ out,
+                                                    const cplx<R>* __restrict__ tw, R scale) {
+    constexpr int NC = N / 2, P = Plan<R, N>::PC, PP = P + 1;
+    __shared__ cplx<R> A[NC * PP], B[NC * PP];
+    __shared__ R O[P * N * NCOMP];
+    const size_t row0 = (size_t)blockIdx.x * P;
+    for (int comp = 0; comp < NCOMP; ++comp) {
+        const cplx<R>* sp = gspec + comp * gcs + row0 * NC;
+        const cplx<R>* ny = gnyq + comp * gcn + row0;
+        for (int t = threadIdx.x; t < P * NC; t += kThreads) {
+            const int c = t / NC, k = t - c * NC;
+            cplx<R> xk = sp[(size_t)c * NC + k];
+            cplx<R> xm = (k == 0) ? ny[c] : sp[(size_t)c * NC + NC - k];
+            if (k == 0) { xk.im = (R)0; xm.im = (R)0; }
+            const R er = xk.re + xm.re, ei = xk.im - xm.im;
+            const R dr = xk.re - xm.re, di = xk.im + xm.im;
+            const cplx<R> w = tw[k];  // exp(-2 pi i k/N); the odd part takes exp(+2 pi i k/N)
+            const R orr = dr * w.re + di * w.im, oi = di * w.re - dr * w.im;
+            A[k * PP + c] = cplx<R>{er - oi, ei + orr};
+        }
+        __syncthreads();
+        const cplx<R>* z = lds_fft<R, NC, P, true, 2>(A, B, tw);
+        for (int t = threadIdx.x; t < P * NC; t += kThreads) {
+            const int c = t / NC, m = t - c * NC;
+            const cplx<R> v = z[m * PP + c];
+            O[(c * N + 2 * m) * NCOMP + comp] = v.re * scale;
+            O[(c * N + 2 * m + 1) * NCOMP + comp] = v.im * scale;
+        }
+        __syncthreads();
+    }
+    R* dst = out + row0 * N * NCOMP;
+    for (int t = threadIdx.x; t < P * N * NCOMP; t += kThreads) dst[t] = O[t];
+}
+
+}  // namespace pm3d

@@ -1,0 +1,326 @@
+// Particle-side kernels of the pm3d step in FP64 (gfx950): CIC key/fraction, int64 fixed-point
+// deposit, CIC gather fused with the kick, kick+drift, and the Lagrangian-lattice Jacobian.
+//
+// Particle arrays are addressed as x[d*sd + i*si]: SoA device state uses (sd, si) = (n, 1), host
+// (n,3) AoS arrays staged for the reference-shaped entry points use (1, 3).
+#pragma once
+#include "common.hpp"
+
+namespace pm3d {
+
+struct Strided {
+    const double* p;
+    long long sd, si;
+    __device__ __forceinline__ double operator()(int d, long long i) const { return p[d * sd + i * si]; }
+};
+
+// ---------------------------------------------------------------------------------------- CIC key
+// _cic_index on the three axes (pm3d.py:76-79): key = (ix0*N+iy0)*N+iz0, frac (n,3).
+__global__ __launch_bounds__(kThreads) void k_cic_index(Strided X, long long n, int N, double dx,
+                                                        long long* __restrict__ key, double* __restrict__ frac) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int lo[3], hi[3];
+    double f[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cic_axis(X(d, i), dx, N, lo[d], hi[d], f[d]);
+    key[i] = ((long long)lo[0] * N + lo[1]) * N + lo[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) frac[i * 3 + d] = f[d];
+}
+
+// ------------------------------------------------------------------------------------ CIC weights
+// The 8 corner weights ((wx*wy)*wz) in the reference's loop order (pm3d.py:85-88): corner
+// c = 4*cx + 2*cy + cz, side 0 weight 1-f, side 1 weight f.
+__device__ __forceinline__ void cic_weights(const double f[3], double w[8]) {
+    const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = (wx[c >> 2] * wy[(c >> 1) & 1]) * wz[c & 1];
+}
+
+// Fixed-point quantisation of one particle's 8 weights with exact unit mass: every corner is
+// rounded to nearest at 2^-FB except the largest-weight corner, which takes the remainder so that
+// the 8 integers sum to exactly 2^FB.  All values stay >= 0 (the largest weight is >= 1/8).
+__device__ __forceinline__ void quantise8(const double f[3], const double w[8], double qs, long long one,
+                                          long long q[8]) {
+    const int cmax = ((f[0] > 0.5) << 2) | ((f[1] > 0.5) << 1) | (f[2] > 0.5);
+    long long sum = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        q[c] = (c == cmax) ? 0 : (long long)rint(w[c] * qs);
+        sum += q[c];
+    }
+    q[cmax] = one - sum;
+}
+
+// ------------------------------------------------------------------------------- deposit (atomic)
+// Direct scatter: each particle adds its 8 fixed-point weights with 64-bit integer atomics.
+// Integer addition commutes, so the mesh is bitwise identical whatever the launch order.
+__global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long long n, int N, double dx,
+                                                             unsigned long long* __restrict__ mesh, double qs,
+                                                             long long one) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int lo[3], hi[3];
+    double f[3], w[8];
+    long long q[8];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cic_axis(X(d, i), dx, N, lo[d], hi[d], f[d]);
+    cic_weights(f, w);
+    quantise8(f, w, qs, one, q);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (q[c] == 0) continue;
+        const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
+        atomicAdd(mesh + ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
+    }
+}
+
+// ------------------------------------------------------------------------------ deposit (tiled)
+// Owner-computes deposit.  The mesh is cut into bricks of BX*BY*BZ cells; the workgroup owning a
+// brick reads every particle whose Lagrangian lattice site lies within HALO sites of the brick
+// (Np == N: site s sits at the centre of cell s), accumulates the corners that land inside its
+// brick in LDS with 64-bit integer atomics, and writes the brick once with plain stores.
+// A particle displaced by HALO cells or more from its site along any axis ("far") is skipped here and
+// appended, by the workgroup owning its site, to `far` for k_deposit_far, which adds it afterwards
+// with global atomics.  The result is bitwise identical to k_deposit_atomic.
+template <int BX, int BY, int BZ, int HALO>
+__global__ __launch_bounds__(kThreads) void k_deposit_brick(const double* __restrict__ x, long long n, int N,
+                                                            double dx, unsigned long long* __restrict__ mesh,
+                                                            double qs, long long one, int* __restrict__ far,
+                                                            unsigned int* __restrict__ nfar) {
+    constexpr int NCELL = BX * BY * BZ;
+    constexpr int EX = BX + 2 * HALO, EY = BY + 2 * HALO, EZ = BZ + 2 * HALO;
+    __shared__ unsigned long long acc[NCELL];
+    for (int t = threadIdx.x; t < NCELL; t += kThreads) acc[t] = 0ull;
+    __syncthreads();
+    const int nbz = N / BZ, nby = N / BY;
+    const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
+    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+    const double* x1 = x + n;
+    const double* x2 = x + 2 * n;
+    for (int t = threadIdx.x; t < EX * EY * EZ; t += kThreads) {
+        const int ez = t % EZ, ey = (t / EZ) % EY, ex = t / (EZ * EY);
+        const int sx = (x0 + ex - HALO + N) % N, sy = (y0 + ey - HALO + N) % N, sz = (z0 + ez - HALO + N) % N;
+        const long long i = ((long long)sx * N + sy) * N + sz;
+        const double X[3] = {x[i], x1[i], x2[i]};
+        int lo[3], hi[3];
+        double f[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cic_axis(X[d], dx, N, lo[d], hi[d], f[d]);
+        // displacement of the base cell from the site, minimum image, per axis
+        const int site[3] = {sx, sy, sz};
+        bool is_far = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int dd = lo[d] - site[d];
+            if (dd > N / 2) dd -= N;
+            if (dd < -N / 2) dd += N;
+            // corners lo, lo+1 must stay within [site-HALO, site+HALO]
+            is_far |= (dd < -HALO) || (dd > HALO - 1);
+        }
+        const bool own_site = ex >= HALO && ex < HALO + BX && ey >= HALO && ey < HALO + BY && ez >= HALO && ez < HALO + BZ;
+        if (is_far) {
+            if (own_site) far[atomicAdd(nfar, 1u)] = (int)i;
+            continue;
+        }
+        double w[8];
+        long long q[8];
+        cic_weights(f, w);
+        quantise8(f, w, qs, one, q);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = ((c >> 2) ? hi[0] : lo[0]) - x0, cy = (((c >> 1) & 1) ? hi[1] : lo[1]) - y0,
+                      cz = ((c & 1) ? hi[2] : lo[2]) - z0;
+            // wrap relative coordinates into the brick frame (brick may touch the periodic seam)
+            const int rx = (cx + N) % N, ry = (cy + N) % N, rz = (cz + N) % N;
+            if (rx < BX && ry < BY && rz < BZ && q[c] != 0)
+                atomicAdd(&acc[(rx * BY + ry) * BZ + rz], (unsigned long long)q[c]);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NCELL; t += kThreads) {
+        const int rz = t % BZ, ry = (t / BZ) % BY, rx = t / (BZ * BY);
+        mesh[((size_t)(x0 + rx) * N + (y0 + ry)) * N + (z0 + rz)] = acc[t];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_deposit_far(const double* __restrict__ x, long long n, int N, double dx,
+                                                          unsigned long long* __restrict__ mesh, double qs,
+                                                          long long one, const int* __restrict__ far,
+                                                          const unsigned int* __restrict__ nfar) {
+    const unsigned int m = *nfar;
+    for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < m; t += gridDim.x * kThreads) {
+        const long long i = far[t];
+        int lo[3], hi[3];
+        double f[3], w[8];
+        long long q[8];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cic_axis(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
+        cic_weights(f, w);
+        quantise8(f, w, qs, one, q);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (q[c] == 0) continue;
+            const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
+            atomicAdd(mesh + ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------- gather
+// acc_c = sum over corners ((g_c*wx)*wy)*wz in the reference's order (pm3d.py:91-99,113), from the
+// interleaved gradient mesh grad[cell*3 + c].  KICK: p += (c1*acc)/den (pm3d.py:127), and the
+// acceleration is always stored for the opening kick of the next step (pm3d.py:128).
+template <bool KICK>
+__global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__ x, long long n, int N, double dx,
+                                                      const double* __restrict__ grad, double* __restrict__ acc,
+                                                      double* __restrict__ p, double c1, double den) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int lo[3], hi[3];
+    double f[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cic_axis(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
+    const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
+        const double* g = grad + (((size_t)ix * N + iy) * N + iz) * 3;
+        const double u = wx[c >> 2], v = wy[(c >> 1) & 1], w = wz[c & 1];
+        a0 = a0 + ((g[0] * u) * v) * w;
+        a1 = a1 + ((g[1] * u) * v) * w;
+        a2 = a2 + ((g[2] * u) * v) * w;
+    }
+    acc[i] = a0;
+    acc[n + i] = a1;
+    acc[2 * n + i] = a2;
+    if (KICK) {
+        p[i] = p[i] + (c1 * a0) / den;
+        p[n + i] = p[n + i] + (c1 * a1) / den;
+        p[2 * n + i] = p[2 * n + i] + (c1 * a2) / den;
+    }
+}
+
+// Scalar-field gather for the interp(F, X) entry point (pm3d.py:91-99).
+__global__ __launch_bounds__(kThreads) void k_gather1(Strided X, long long n, int N, double dx,
+                                                      const double* __restrict__ F, double* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    int lo[3], hi[3];
+    double f[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cic_axis(X(d, i), dx, N, lo[d], hi[d], f[d]);
+    const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
+    double a = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
+        a = a + ((F[((size_t)ix * N + iy) * N + iz] * wx[c >> 2]) * wy[(c >> 1) & 1]) * wz[c & 1];
+    }
+    out[i] = a;
+}
+
+// ---------------------------------------------------------------------------------- kick + drift
+// Opening half kick with the cached acceleration, p += (c1*acc)/den1 (pm3d.py:122), then the drift
+// x = (x + (da*p)/den_d) % L (pm3d.py:123-124).  Non-finite results are counted in *bad.
+// With `pend`, the previous step's deferred closing kick p += (c1p*acc)/denp (pm3d.py:127) is
+// applied first, as a separate rounding.
+__global__ __launch_bounds__(kThreads) void k_kick_drift(double* __restrict__ x, double* __restrict__ p,
+                                                         const double* __restrict__ acc, long long n, int pend,
+                                                         double c1p, double denp, double c1, double den1, double da,
+                                                         double den_d, double L, unsigned long long* __restrict__ bad) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    bool ok = true;
+    if (i < n) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const long long k = d * n + i;
+            const double ak = acc[k];
+            double pd = p[k];
+            if (pend) pd = pd + (c1p * ak) / denp;
+            pd = pd + (c1 * ak) / den1;
+            const double xd = npy_rem(x[k] + (da * pd) / den_d, L);
+            p[k] = pd;
+            x[k] = xd;
+            ok &= isfinite(pd) && isfinite(xd);
+        }
+    }
+    const unsigned long long nb = __ballot(!ok);
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, (unsigned long long)__popcll(nb));
+}
+
+// ----------------------------------------------------------------------------------- utilities
+__global__ __launch_bounds__(kThreads) void k_fill_u64(unsigned long long* __restrict__ a, long long n,
+                                                       unsigned long long v) {
+    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
+        a[i] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_fixed_to_real(const long long* __restrict__ q, long long m, double qscale,
+                                                            double* __restrict__ out) {
+    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < m; i += (long long)gridDim.x * kThreads)
+        out[i] = (double)q[i] * qscale;
+}
+
+// delta = (q*2^-FB)/n_mean - 1 (pm3d.py:104) of the last deposit, for a lazy host read.
+__global__ __launch_bounds__(kThreads) void k_fixed_to_delta(const long long* __restrict__ q, long long m, double qscale,
+                                                             double nmean, double* __restrict__ out) {
+    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < m; i += (long long)gridDim.x * kThreads)
+        out[i] = ((double)q[i] * qscale) / nmean - 1.0;
+}
+
+// (n,3) AoS <-> SoA copies of FP64 particle arrays.
+__global__ __launch_bounds__(kThreads) void k_aos_to_soa(const double* __restrict__ a, double* __restrict__ s, long long n) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d * n + i] = a[i * 3 + d];
+}
+__global__ __launch_bounds__(kThreads) void k_soa_to_aos(const double* __restrict__ s, double* __restrict__ a, long long n) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[i * 3 + d] = s[d * n + i];
+}
+
+// Closing half kick applied lazily before the host reads p: p += (c1*acc)/den.
+__global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ p, const double* __restrict__ acc, long long n3,
+                                                   double c1, double den) {
+    const long long k = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (k < n3) p[k] = p[k] + (c1 * acc[k]) / den;
+}
+
+// ---------------------------------------------------------------------------- shell crossing
+// min over Lagrangian lattice cells of det(dx/dq): forward differences to the +x, +y, +z lattice
+// neighbours, minimum-image unwrapped, divided by the lattice spacing dq.  Folded into *jmin
+// through the order-preserving int64 image of the double.
+__global__ __launch_bounds__(kThreads) void k_min_jacobian(const double* __restrict__ x, int Np, double L, double dq,
+                                                           long long* __restrict__ jmin) {
+    const long long n = (long long)Np * Np * Np;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    double det = 1e300;
+    if (i < n) {
+        const int a = (int)(i / ((long long)Np * Np)), b = (int)((i / Np) % Np), c = (int)(i % Np);
+        const long long nb[3] = {((long long)((a + 1) % Np) * Np + b) * Np + c,
+                                 ((long long)a * Np + (b + 1) % Np) * Np + c,
+                                 ((long long)a * Np + b) * Np + (c + 1) % Np};
+        double J[3][3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double v = x[d * n + nb[e]] - x[d * n + i];
+                v = v - L * rint(v / L);
+                J[e][d] = v / dq;
+            }
+        }
+        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    }
+    long long o = wave_min_i64(ordered_bits(det));
+    if ((threadIdx.x & 63) == 0) atomicMin(jmin, o);
+}
+
+}  // namespace pm3d

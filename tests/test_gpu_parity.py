@@ -1,0 +1,260 @@
+"""GPU parity of the HIP path against the CPU FP64 oracle (oracle/pm3d_oracle.py) and the reference's
+golden vectors.  Every call goes through the C-ABI of _pm3d_hip.so.
+
+Tolerances (FP64 mode): cell keys and CIC fractions bit-exact; everything else
+max|gpu - ref| / max|ref| <= 1e-10 per field (positions: minimum-image difference over L).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pm3d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+
+def pos_err(x, y, L):
+    d = (np.asarray(x) - np.asarray(y) + L / 2) % L - L / 2
+    return float(np.abs(d).max() / L)
+
+
+@pytest.fixture(scope="module")
+def PM3D():
+    from pbh_amd.pm3d import PM3D as cls
+    return cls
+
+
+def lattice(N, L, jitter=0.0, seed=0):
+    q = (np.arange(N) + 0.5) * (L / N)
+    X = np.stack(np.meshgrid(q, q, q, indexing="ij"), -1).reshape(-1, 3)
+    if jitter:
+        X = (X + jitter * (L / N) * np.random.default_rng(seed).standard_normal(X.shape)) % L
+    return X
+
+
+# ------------------------------------------------------------------------------ CIC keys
+@pytest.mark.parametrize("N,L", [(16, 1.0), (128, 1.0), (64, 8 * np.sqrt(6.0))])
+def test_cic_keys_bit_exact(PM3D, N, L):
+    say("cic keys", N, L)
+    pm = PM3D(N, L, 8)
+    dx = L / N
+    rng = np.random.default_rng(N)
+    X = rng.uniform(0, L, (200000, 3))
+    edge = np.array([0.0, L, np.nextafter(L, 0), 0.5 * dx, np.nextafter(0.5 * dx, 0), np.nextafter(0.5 * dx, 1),
+                     np.remainder(-1e-20, L), np.remainder(-0.0, L), 3.5 * dx, (N - 0.5) * dx])
+    X[: len(edge), 0] = edge
+    X[: len(edge), 1] = edge[::-1]
+    X[: len(edge), 2] = np.roll(edge, 3)
+    key, frac = pm.cic_keys(X)
+    assert np.array_equal(key, O.cic_key(X, dx, N))
+    ref = np.stack([O.cic_index(X[:, d], dx, N)[2] for d in range(3)], -1)
+    assert np.array_equal(frac, ref)
+
+
+# ------------------------------------------------------------------------------- deposit
+@pytest.mark.parametrize("N,n", [(8, 1), (16, 5000), (64, 300000), (128, 2097152)])
+def test_deposit_parity(PM3D, N, n):
+    say("deposit", N, n)
+    L = 1.0
+    pm = PM3D(N, L, 8)
+    X = np.random.default_rng(n).uniform(0, L, (n, 3))
+    rho = pm.deposit(X)
+    ref = O.deposit(X, L / N, N)
+    assert rel(rho, ref) <= TOL
+    assert abs(rho.sum() - n) <= 1e-9 * n
+
+
+def test_deposit_lattice_exact(PM3D):
+    say("deposit lattice")
+    N = 32
+    pm = PM3D(N, 1.0, N)
+    assert np.array_equal(pm.deposit(lattice(N, 1.0)), np.ones((N, N, N)))
+    assert np.array_equal(pm.deposit(lattice(2 * N, 1.0)), 8 * np.ones((N, N, N)))
+    assert np.array_equal(pm.deposit(np.zeros((0, 3))), np.zeros((N, N, N)))
+
+
+# -------------------------------------------------------------------------------- interp
+@pytest.mark.parametrize("N", [8, 32, 128])
+def test_interp_parity(PM3D, N):
+    say("interp", N)
+    L = 2.0
+    pm = PM3D(N, L, 8)
+    rng = np.random.default_rng(N + 1)
+    F = rng.standard_normal((N, N, N))
+    X = rng.uniform(0, L, (100000, 3))
+    assert rel(pm.interp(F, X), O.interp(F, X, L / N, N)) <= 1e-14
+
+
+# ----------------------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("N", [8, 16, 32, 64, 128, 256])
+def test_rfftn_parity(PM3D, N):
+    say("rfftn", N)
+    pm = PM3D(N, 1.0, 8)
+    a = np.random.default_rng(N).standard_normal((N, N, N))
+    assert rel(pm.rfftn(a), np.fft.rfftn(a)) <= 1e-13
+
+
+@pytest.mark.parametrize("N", [8, 16, 64, 128])
+def test_irfftn_parity_non_hermitian(PM3D, N):
+    """numpy's c2r drops the imaginary parts of the kz=0 and kz=N/2 bins after the x,y inverse
+    transforms; random (non-Hermitian) input exercises exactly that convention."""
+    say("irfftn", N)
+    pm = PM3D(N, 1.0, 8)
+    rng = np.random.default_rng(N + 7)
+    c = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    assert rel(pm.irfftn(c), np.fft.irfftn(c, s=(N, N, N), axes=(0, 1, 2))) <= 1e-13
+
+
+@pytest.mark.parametrize("N,smooth", [(16, 0.0), (64, 2.0), (128, 0.0)])
+def test_poisson_gradient_parity(PM3D, N, smooth):
+    say("poisson", N, smooth)
+    L, a = 3.0, 1.3
+    pm = PM3D(N, L, 8, smooth=smooth)
+    ref = O.OraclePM3D(N, L, 8, smooth=smooth)
+    delta = np.random.default_rng(N).standard_normal((N, N, N))
+    g = pm.poisson_gradient(delta, a)
+    gr = ref.gradient_meshes(ref.potential_k(delta, a))
+    for c in range(3):
+        assert rel(g[..., c], gr[c]) <= 1e-12, c
+
+
+# ------------------------------------------------------------- reference golden vectors
+def _spline():
+    from pbh_amd.ltb import profile_table
+    return profile_table("yoo")["spline"]
+
+
+def test_make_ic_matches_golden(PM3D, golden):
+    arrays, meta = golden
+    for c in meta["cases"]:
+        say("make_ic", c["name"])
+        pm = PM3D(c["N"], c["L"], c["Np"], seed=c["seed"], smooth=c["smooth"]).make_ic(
+            c["mu"], _spline(), c["e"], c["p"], c["tail"], (2.0, 10.0))
+        assert pos_err(pm.x, arrays[c["name"] + "/x0"], c["L"]) <= TOL
+        assert rel(pm.p, arrays[c["name"] + "/p0"]) <= TOL
+        assert rel(pm.delta_lin, arrays[c["name"] + "/delta_lin"]) <= TOL
+        assert pm.tail_rms_grid == pytest.approx(c["tail_rms_grid"], rel=1e-12)
+
+
+def test_forces_and_steps_match_golden(PM3D, golden):
+    arrays, meta = golden
+    for c in meta["cases"]:
+        nm = c["name"]
+        say("golden steps", nm)
+        pm = PM3D(c["N"], c["L"], c["Np"], seed=c["seed"], smooth=c["smooth"])
+        pm.set_state(arrays[nm + "/x0"], arrays[nm + "/p0"])
+        acc, delta = pm.forces(1.0)
+        assert rel(acc, arrays[nm + "/acc0"]) <= TOL
+        assert rel(delta, arrays[nm + "/delta0"]) <= TOL
+        a = 1.0
+        for k, st in enumerate(c["steps"]):
+            a, dm = pm.step(a, st["da"])
+            assert a == st["a"]
+            if f"{nm}/x{k + 1}" in arrays:
+                assert pos_err(pm.x, arrays[f"{nm}/x{k + 1}"], c["L"]) <= TOL
+                assert rel(pm.p, arrays[f"{nm}/p{k + 1}"]) <= TOL
+                ref_delta = arrays[f"{nm}/delta{k + 1}"]
+                assert rel(np.asarray(dm), ref_delta) <= TOL
+                assert dm.max() == pytest.approx(ref_delta.max(), rel=1e-10, abs=1e-10)
+
+
+def test_analyse_matches_golden(PM3D, golden):
+    from pbh_amd.pm3d import analyse
+    arrays, meta = golden
+    for c in meta["cases"]:
+        nm = c["name"]
+        K = c["nsteps"]
+        say("analyse", nm)
+        pm = PM3D(c["N"], c["L"], c["Np"], seed=c["seed"], smooth=c["smooth"]).make_ic(
+            c["mu"], _spline(), c["e"], c["p"], c["tail"], (2.0, 10.0))
+        pm.set_state(arrays[f"{nm}/x{K}"], arrays[f"{nm}/p{K}"])
+        out = analyse(pm, c["steps"][-1]["a"])
+        for key, val in c["analyse"].items():
+            np.testing.assert_allclose(np.asarray(out[key]), np.asarray(val), rtol=1e-9, atol=1e-12, err_msg=key)
+
+
+# ------------------------------------------------------------ trajectories on the workloads
+def _trajectory_case(PM3D, spec, nsteps, da_frac):
+    from pbh_amd import ic
+    st = ic.make(spec)
+    L = spec.L
+    pm = PM3D(spec.N, L, spec.n_side)
+    pm.set_state(st["x"], st["p"])
+    ref = O.OraclePM3D(spec.N, L, spec.n_side)
+    ref.x, ref.p = st["x"].copy(), st["p"].copy()
+    a = st["a_i"]
+    # keys bit-exact on identical positions
+    key, _ = pm.cic_keys(st["x"])
+    assert np.array_equal(key, O.cic_key(st["x"], L / spec.N, spec.N))
+    for k in range(nsteps):
+        da = da_frac * a
+        a_g, dm = pm.step(a, da)
+        a_r, dr = ref.step(a, da)
+        assert a_g == a_r
+        a = a_r
+    assert rel(np.asarray(dm), dr) <= TOL
+    assert pos_err(pm.x, ref.x, L) <= TOL
+    assert rel(pm.p, ref.p) <= TOL
+    return pm, ref
+
+
+def test_trajectory_peak32_parity(PM3D):
+    from pbh_amd import ic
+    say("trajectory peak32")
+    spec = ic.PeakSpec(N=32, mu=0.3, sigma=(0.12, 0.11, 0.10), seed=0)
+    pm, ref = _trajectory_case(PM3D, spec, 40, 0.02)
+    assert pm.min_jacobian() == pytest.approx(O.min_jacobian(ref.x, 32, 1.0), rel=1e-9)
+
+
+def test_trajectory_peak128_parity(PM3D):
+    """128^3 mesh, 2.1M particles, FP64 on the GPU vs the CPU oracle (3 steps from the IC)."""
+    from pbh_amd import ic
+    say("trajectory peak128")
+    _trajectory_case(PM3D, ic.named("peak128"), 3, 0.0125)
+
+
+def test_step_is_deterministic(PM3D):
+    from pbh_amd import ic
+    say("determinism")
+    st = ic.make(ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10)))
+    outs = []
+    for _ in range(2):
+        pm = PM3D(64, 1.0, 64)
+        pm.set_state(st["x"], st["p"])
+        a = 0.01
+        for k in range(5):
+            a, dm = pm.step(a, 0.02 * a)
+        outs.append((pm.x, pm.p, np.asarray(dm)))
+    for u, v in zip(*outs):
+        assert np.array_equal(u, v)
+
+
+def test_tiled_deposit_equals_atomic(PM3D):
+    """The step's owner-computes deposit and the reference-shaped atomic deposit give the same mesh."""
+    from pbh_amd import ic
+    say("tiled vs atomic")
+    spec = ic.PeakSpec(N=64, mu=0.5, sigma=(0.08, 0.07, 0.06))
+    st = ic.make(spec)
+    pm = PM3D(64, 1.0, 64)
+    pm.set_state(st["x"], st["p"])
+    a = 0.01
+    for k in range(60):          # run well into the non-linear regime so that 'far' particles exist
+        a, dm = pm.step(a, 0.03 * a)
+    d_step = np.asarray(dm)
+    rho = pm.deposit(pm.x)
+    assert np.array_equal(d_step, rho / 1.0 - 1.0)
