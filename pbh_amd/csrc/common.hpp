@@ -66,10 +66,53 @@ __device__ __forceinline__ long long wave_min_i64(long long v) {
     }
     return v;
 }
+// Block-wide max of a non-negative double over kThreads threads (all threads must call).
+__device__ __forceinline__ double block_max_nonneg(double v) {
+    __shared__ unsigned long long wmax[kThreads / 64];
+    const unsigned long long bits = wave_max_u64((unsigned long long)__double_as_longlong(v));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    unsigned long long m = wmax[0];
+    for (int w = 1; w < kThreads / 64; ++w) m = wmax[w] > m ? wmax[w] : m;
+    __syncthreads();
+    return __longlong_as_double((long long)m);
+}
+
 // Order-preserving map double -> int64 (for atomicMin on doubles of any sign).
 __device__ __forceinline__ long long ordered_bits(double v) {
     long long i = __double_as_longlong(v);
     return i >= 0 ? i : (i ^ 0x7fffffffffffffffLL);
+}
+
+// ----------------------------------------------------------------- deterministic fixed point
+// Cell accumulators of the deposit.  Integer addition commutes, so every launch order gives the
+// same bits.
+//   narrow: one uint64 word per cell, value = q * 2^-FB
+//   wide  : two uint64 words {lo, hi} per cell, value = (hi*2^64 + lo) * 2^-60; the carry out of lo
+//           is detected from the value atomicAdd returns, and the number of carries equals
+//           floor(sum/2^64) whatever the order.  Quantum 2^-60 ~ 8.7e-19 of a particle, so the FP64
+//           mode is exact to FP64 rounding even where delta = rho - 1 is tiny.
+constexpr int kWideFB = 60;
+
+template <bool WIDE>
+__device__ __forceinline__ void fx_atomic_add(unsigned long long* mesh, size_t cell, unsigned long long q) {
+    if constexpr (WIDE) {
+        unsigned long long* c = mesh + 2 * cell;
+        const unsigned long long old = atomicAdd(c, q);
+        if (old + q < old) atomicAdd(c + 1, 1ull);
+    } else {
+        atomicAdd(mesh + cell, q);
+    }
+}
+
+template <bool WIDE>
+__device__ __forceinline__ double fx_value(const unsigned long long* mesh, size_t cell, double qscale) {
+    if constexpr (WIDE) {
+        const ulonglong2 v = *(const ulonglong2*)(mesh + 2 * cell);
+        return (double)v.y * 16.0 + (double)v.x * 0x1p-60;
+    } else {
+        return (double)mesh[cell] * qscale;
+    }
 }
 
 }  // namespace pm3d

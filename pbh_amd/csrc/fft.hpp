@@ -91,30 +91,31 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R
 }
 
 // ------------------------------------------------------------------------------------------ z r2c
-// Rows (x,y) of the real mesh -> half spectrum along kz.  FIXED: the input is the int64 fixed-point
-// deposit, converted on the fly to delta = rho/n_mean - 1 (pm3d.py:103-104), with the row maxima
-// folded into *qmax and delta optionally written out.
-template <typename R, int N, bool FIXED>
+// Rows (x,y) of the real mesh -> half spectrum along kz.  FIXED: the input is the fixed-point
+// deposit (WIDE: 128-bit cells), converted on the fly to delta = rho/n_mean - 1 (pm3d.py:103-104),
+// with each block's maximum rho stored to bmax[blockIdx.x] (reduced by the following y pass, so no
+// two blocks contend for one address) and delta optionally written out.  Block 0 also clears the
+// deposit's far-particle counter for the next deposit.
+template <typename R, int N, bool FIXED, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                     cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
-                                                    R* __restrict__ delta_out, unsigned long long* __restrict__ qmax,
-                                                    double qscale, double nmean) {
+                                                    R* __restrict__ delta_out, double* __restrict__ bmax,
+                                                    unsigned long long* __restrict__ nfar, double qscale,
+                                                    double nmean) {
     constexpr int NC = N / 2, P = Plan<R, N>::PZ, PP = P + 1;
     __shared__ cplx<R> A[NC * PP], B[NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
-    unsigned long long lmax = 0;
+    double lmax = 0.0;
     for (int t = threadIdx.x; t < P * NC; t += kThreads) {
         const int c = t / NC, m = t - c * NC;
         const size_t g = (row0 + c) * N + 2 * m;
         R v0, v1;
         if constexpr (FIXED) {
-            const long long q0 = ((const long long*)in)[g], q1 = ((const long long*)in)[g + 1];
-            const unsigned long long u0 = q0 > 0 ? (unsigned long long)q0 : 0ull;
-            const unsigned long long u1 = q1 > 0 ? (unsigned long long)q1 : 0ull;
-            lmax = u0 > lmax ? u0 : lmax;
-            lmax = u1 > lmax ? u1 : lmax;
-            v0 = (R)(((double)q0 * qscale) / nmean - 1.0);
-            v1 = (R)(((double)q1 * qscale) / nmean - 1.0);
+            const unsigned long long* mesh = (const unsigned long long*)in;
+            const double r0 = fx_value<WIDE>(mesh, g, qscale), r1 = fx_value<WIDE>(mesh, g + 1, qscale);
+            lmax = fmax(lmax, fmax(r0, r1));
+            v0 = (R)(r0 / nmean - 1.0);
+            v1 = (R)(r1 / nmean - 1.0);
             if (delta_out) {
                 delta_out[g] = v0;
                 delta_out[g + 1] = v1;
@@ -126,8 +127,11 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
         A[m * PP + c] = cplx<R>{v0, v1};
     }
     if constexpr (FIXED) {
-        lmax = wave_max_u64(lmax);
-        if ((threadIdx.x & 63) == 0 && lmax) atomicMax(qmax, lmax);
+        const double m = block_max_nonneg(lmax);
+        if (threadIdx.x == 0) {
+            bmax[blockIdx.x] = m;
+            if (blockIdx.x == 0 && nfar) *nfar = 0ull;
+        }
     }
     __syncthreads();
     const cplx<R>* Z = lds_fft<R, NC, P, false, 2>(A, B, tw);
@@ -146,11 +150,21 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
 
 // ------------------------------------------------------------------------------------------ y c2c
 // In-place complex FFT along y on `ncomp` = gridDim.y half spectra (component strides cs, cn).
+// If nbmax > 0, the last block also reduces the r2c pass's per-block maxima bmax[0..nbmax) into
+// *slot (the deposit's maximum rho, as the bits of a non-negative double).
 template <typename R, int N, bool INV>
 __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
-                                                    size_t cs, size_t cn, const cplx<R>* __restrict__ tw) {
+                                                    size_t cs, size_t cn, const cplx<R>* __restrict__ tw,
+                                                    const double* __restrict__ bmax, int nbmax,
+                                                    unsigned long long* __restrict__ slot) {
     constexpr int P = Plan<R, N>::PY, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
     __shared__ cplx<R> A[N * PP], B[N * PP];
+    if (nbmax > 0 && blockIdx.x == gridDim.x - 1 && blockIdx.y == 0) {
+        double m = 0.0;
+        for (int t = threadIdx.x; t < nbmax; t += kThreads) m = fmax(m, bmax[t]);
+        m = block_max_nonneg(m);
+        if (threadIdx.x == 0) *slot = (unsigned long long)__double_as_longlong(m);
+    }
     spec += blockIdx.y * cs;
     nyq += blockIdx.y * cn;
     const int b = blockIdx.x;

@@ -41,6 +41,7 @@ __device__ __forceinline__ void cic_weights(const double f[3], double w[8]) {
 // Fixed-point quantisation of one particle's 8 weights with exact unit mass: every corner is
 // rounded to nearest at 2^-FB except the largest-weight corner, which takes the remainder so that
 // the 8 integers sum to exactly 2^FB.  All values stay >= 0 (the largest weight is >= 1/8).
+// FB <= 60, so every value fits in 63 bits.
 __device__ __forceinline__ void quantise8(const double f[3], const double w[8], double qs, long long one,
                                           long long q[8]) {
     const int cmax = ((f[0] > 0.5) << 2) | ((f[1] > 0.5) << 1) | (f[2] > 0.5);
@@ -56,6 +57,7 @@ __device__ __forceinline__ void quantise8(const double f[3], const double w[8], 
 // ------------------------------------------------------------------------------- deposit (atomic)
 // Direct scatter: each particle adds its 8 fixed-point weights with 64-bit integer atomics.
 // Integer addition commutes, so the mesh is bitwise identical whatever the launch order.
+template <bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long long n, int N, double dx,
                                                              unsigned long long* __restrict__ mesh, double qs,
                                                              long long one) {
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
     for (int c = 0; c < 8; ++c) {
         if (q[c] == 0) continue;
         const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-        atomicAdd(mesh + ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
+        fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
     }
 }
 
@@ -84,22 +86,25 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
 // A particle displaced by HALO cells or more from its site along any axis ("far") is skipped here and
 // appended, by the workgroup owning its site, to `far` for k_deposit_far, which adds it afterwards
 // with global atomics.  The result is bitwise identical to k_deposit_atomic.
-template <int BX, int BY, int BZ, int HALO>
-__global__ __launch_bounds__(kThreads) void k_deposit_brick(const double* __restrict__ x, long long n, int N,
-                                                            double dx, unsigned long long* __restrict__ mesh,
-                                                            double qs, long long one, int* __restrict__ far,
-                                                            unsigned int* __restrict__ nfar) {
+template <int BX, int BY, int BZ, int HALO, bool WIDE, int NT>
+__global__ __launch_bounds__(NT) void k_deposit_brick(const double* __restrict__ x, long long n, int N, double dx,
+                                                      unsigned long long* __restrict__ mesh, double qs, long long one,
+                                                      int* __restrict__ far, unsigned int* __restrict__ nfar) {
     constexpr int NCELL = BX * BY * BZ;
     constexpr int EX = BX + 2 * HALO, EY = BY + 2 * HALO, EZ = BZ + 2 * HALO;
     __shared__ unsigned long long acc[NCELL];
-    for (int t = threadIdx.x; t < NCELL; t += kThreads) acc[t] = 0ull;
+    __shared__ unsigned int carry[WIDE ? NCELL : 1];
+    for (int t = threadIdx.x; t < NCELL; t += NT) {
+        acc[t] = 0ull;
+        if (WIDE) carry[t] = 0u;
+    }
     __syncthreads();
     const int nbz = N / BZ, nby = N / BY;
     const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
     const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
     const double* x1 = x + n;
     const double* x2 = x + 2 * n;
-    for (int t = threadIdx.x; t < EX * EY * EZ; t += kThreads) {
+    for (int t = threadIdx.x; t < EX * EY * EZ; t += NT) {
         const int ez = t % EZ, ey = (t / EZ) % EY, ex = t / (EZ * EY);
         const int sx = (x0 + ex - HALO + N) % N, sy = (y0 + ey - HALO + N) % N, sz = (z0 + ez - HALO + N) % N;
         const long long i = ((long long)sx * N + sy) * N + sz;
@@ -120,10 +125,18 @@ __global__ __launch_bounds__(kThreads) void k_deposit_brick(const double* __rest
             is_far |= (dd < -HALO) || (dd > HALO - 1);
         }
         const bool own_site = ex >= HALO && ex < HALO + BX && ey >= HALO && ey < HALO + BY && ez >= HALO && ez < HALO + BZ;
-        if (is_far) {
-            if (own_site) far[atomicAdd(nfar, 1u)] = (int)i;
-            continue;
+        // append owned far particles: one atomic per wave, slots by prefix popcount
+        const bool app = is_far && own_site;
+        const unsigned long long bal = __ballot(app);
+        if (bal) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)bal) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(nfar, (unsigned int)__popcll(bal));
+            base = __shfl(base, leader, 64);
+            if (app) far[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)i;
         }
+        if (is_far) continue;
         double w[8];
         long long q[8];
         cic_weights(f, w);
@@ -134,17 +147,25 @@ __global__ __launch_bounds__(kThreads) void k_deposit_brick(const double* __rest
                       cz = ((c & 1) ? hi[2] : lo[2]) - z0;
             // wrap relative coordinates into the brick frame (brick may touch the periodic seam)
             const int rx = (cx + N) % N, ry = (cy + N) % N, rz = (cz + N) % N;
-            if (rx < BX && ry < BY && rz < BZ && q[c] != 0)
-                atomicAdd(&acc[(rx * BY + ry) * BZ + rz], (unsigned long long)q[c]);
+            if (rx < BX && ry < BY && rz < BZ && q[c] != 0) {
+                const int cell = (rx * BY + ry) * BZ + rz;
+                const unsigned long long old = atomicAdd(&acc[cell], (unsigned long long)q[c]);
+                if (WIDE && old + (unsigned long long)q[c] < old) atomicAdd(&carry[cell], 1u);
+            }
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < NCELL; t += kThreads) {
+    for (int t = threadIdx.x; t < NCELL; t += NT) {
         const int rz = t % BZ, ry = (t / BZ) % BY, rx = t / (BZ * BY);
-        mesh[((size_t)(x0 + rx) * N + (y0 + ry)) * N + (z0 + rz)] = acc[t];
+        const size_t cell = ((size_t)(x0 + rx) * N + (y0 + ry)) * N + (z0 + rz);
+        if (WIDE)
+            *(ulonglong2*)(mesh + 2 * cell) = make_ulonglong2(acc[t], (unsigned long long)carry[t]);
+        else
+            mesh[cell] = acc[t];
     }
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_deposit_far(const double* __restrict__ x, long long n, int N, double dx,
                                                           unsigned long long* __restrict__ mesh, double qs,
                                                           long long one, const int* __restrict__ far,
@@ -163,7 +184,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far(const double* __restri
         for (int c = 0; c < 8; ++c) {
             if (q[c] == 0) continue;
             const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-            atomicAdd(mesh + ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
+            fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
         }
     }
 }
@@ -258,17 +279,15 @@ __global__ __launch_bounds__(kThreads) void k_fill_u64(unsigned long long* __res
         a[i] = v;
 }
 
-__global__ __launch_bounds__(kThreads) void k_fixed_to_real(const long long* __restrict__ q, long long m, double qscale,
+// rho of a fixed-point mesh; with `as_delta`, delta = rho/n_mean - 1 (pm3d.py:104).
+template <bool WIDE>
+__global__ __launch_bounds__(kThreads) void k_fixed_to_real(const unsigned long long* __restrict__ q, long long m,
+                                                            double qscale, int as_delta, double nmean,
                                                             double* __restrict__ out) {
-    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < m; i += (long long)gridDim.x * kThreads)
-        out[i] = (double)q[i] * qscale;
-}
-
-// delta = (q*2^-FB)/n_mean - 1 (pm3d.py:104) of the last deposit, for a lazy host read.
-__global__ __launch_bounds__(kThreads) void k_fixed_to_delta(const long long* __restrict__ q, long long m, double qscale,
-                                                             double nmean, double* __restrict__ out) {
-    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < m; i += (long long)gridDim.x * kThreads)
-        out[i] = ((double)q[i] * qscale) / nmean - 1.0;
+    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < m; i += (long long)gridDim.x * kThreads) {
+        const double r = fx_value<WIDE>(q, i, qscale);
+        out[i] = as_delta ? r / nmean - 1.0 : r;
+    }
 }
 
 // (n,3) AoS <-> SoA copies of FP64 particle arrays.

@@ -55,7 +55,8 @@ struct Handle {
     // particle state (FP64 SoA)
     double *x = nullptr, *p = nullptr, *acc = nullptr;
     // meshes
-    unsigned long long* mesh = nullptr;  // int64 fixed-point deposit
+    unsigned long long* mesh = nullptr;  // fixed-point deposit: FP64 mode {lo,hi} per cell (2^-60)
+    bool wide = true;
     cplx<double>* spec = nullptr;        // [N][N][N/2] + nyq [N][N]
     cplx<double>* gspec = nullptr;       // 3 x the same
     double* grad = nullptr;              // [M][3]
@@ -66,6 +67,7 @@ struct Handle {
     unsigned long long* dscal = nullptr;
     long long serial = 0;                // number of deposits of the state so far
     int* far = nullptr;                  // far-particle list of the tiled deposit
+    double* bmax = nullptr;              // per-block maxima of the r2c pass
     // staging for reference-shaped entry points
     double* sbuf = nullptr;
     size_t sbuf_bytes = 0;
@@ -144,15 +146,18 @@ int launch_fft_forward_tmpl(Handle* h, const void* in, bool fixed, double* delta
         Timed t(h, KC_R2C_Z);
         const unsigned g = (unsigned)((size_t)N * N / Plan<R, N>::PZ);
         if (fixed)
-            k_r2c_z<R, N, true><<<g, kThreads, 0, h->st>>>(in, spec, nyq, h->tw, delta_out, qmax_slot(h), h->qscale, nmean);
+            k_r2c_z<R, N, true, true><<<g, kThreads, 0, h->st>>>(in, spec, nyq, h->tw, delta_out, h->bmax, h->dscal,
+                                                                 h->qscale, nmean);
         else
-            k_r2c_z<R, N, false><<<g, kThreads, 0, h->st>>>(in, spec, nyq, h->tw, nullptr, nullptr, 0.0, 1.0);
+            k_r2c_z<R, N, false, true><<<g, kThreads, 0, h->st>>>(in, spec, nyq, h->tw, nullptr, nullptr, nullptr, 0.0,
+                                                                  1.0);
     }
     {
         Timed t(h, KC_Y_FWD);
         constexpr int P = Plan<R, N>::PY;
         dim3 g(N * (N / 2 / P) + N / P, 1);
-        k_c2c_y<R, N, false><<<g, kThreads, 0, h->st>>>(spec, nyq, 0, 0, h->tw);
+        const int nb = fixed ? (int)((size_t)N * N / Plan<R, N>::PZ) : 0;
+        k_c2c_y<R, N, false><<<g, kThreads, 0, h->st>>>(spec, nyq, 0, 0, h->tw, h->bmax, nb, qmax_slot(h));
     }
     return 0;
 }
@@ -184,7 +189,7 @@ int launch_inverse_tmpl(Handle* h, cplx<double>* base, int ncomp, double* out) {
         Timed t(h, KC_Y_INV);
         constexpr int P = Plan<R, N>::PY;
         dim3 g(N * (N / 2 / P) + N / P, ncomp);
-        k_c2c_y<R, N, true><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, h->tw);
+        k_c2c_y<R, N, true><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, h->tw, nullptr, 0, nullptr);
     }
     {
         Timed t(h, KC_C2R_Z);
@@ -232,30 +237,30 @@ int ensure_sbuf(Handle* h, size_t bytes) {
 int deposit_state(Handle* h) {
     // bricks of 16^3 cells with a one-site halo: the halo must not wrap onto the brick itself
     const bool tiled = h->deposit_mode == 1 && h->Np == h->N && h->N >= 32;
+    // nfar is cleared by the r2c pass of the previous deposit (and at creation); the maximum of
+    // this deposit is stored (not accumulated) into its ring slot by the y pass
     h->serial += 1;
-    {
+    if (!tiled) {
         Timed t(h, KC_FILL);
-        if (!tiled) k_fill_u64<<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>(h->mesh, h->M, 0ull);
-        k_fill_u64<<<1, kThreads, 0, h->st>>>(h->dscal, 1, 0ull);                      // nfar
-        k_fill_u64<<<1, kThreads, 0, h->st>>>(qmax_slot(h), 1, 0ull);                  // this deposit's max
+        k_fill_u64<<<std::min<long long>(grid_for(2 * h->M), 4096), kThreads, 0, h->st>>>(h->mesh, 2 * h->M, 0ull);
     }
     if (tiled) {
         {
             Timed t(h, KC_DEPOSIT);
             const int B = 16;
             const unsigned g = (unsigned)((h->N / B) * (h->N / B) * (h->N / B));
-            k_deposit_brick<16, 16, 16, 1><<<g, kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone,
-                                                                      h->far, (unsigned int*)(h->dscal + 0));
+            k_deposit_brick<16, 16, 16, 1, true, 1024><<<g, 1024, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs,
+                                                                               h->qone, h->far, (unsigned int*)(h->dscal + 0));
         }
         {
             Timed t(h, KC_DEPOSIT_FAR);
-            k_deposit_far<<<1024, kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone, h->far,
+            k_deposit_far<true><<<1024, kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone, h->far,
                                                         (const unsigned int*)(h->dscal + 0));
         }
     } else {
         Timed t(h, KC_DEPOSIT);
         Strided X{h->x, h->n, 1};
-        k_deposit_atomic<<<grid_for(h->n), kThreads, 0, h->st>>>(X, h->n, h->N, h->dx, h->mesh, h->qs, h->qone);
+        k_deposit_atomic<true><<<grid_for(h->n), kThreads, 0, h->st>>>(X, h->n, h->N, h->dx, h->mesh, h->qs, h->qone);
     }
     return check_launch(h, "deposit");
 }
@@ -302,6 +307,7 @@ void free_all(Handle* h) {
     if (h->tw) hipFree(h->tw);
     if (h->dscal) hipFree(h->dscal);
     if (h->far) hipFree(h->far);
+    if (h->bmax) hipFree(h->bmax);
     for (auto& u : h->ev_used) {
         hipEventDestroy(u.second.first);
         hipEventDestroy(u.second.second);
@@ -348,9 +354,10 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         g_create_error = "precision mode not built";
         return PM3D_EUNSUPPORTED;
     }
-    const int fb = cfg->frac_bits ? cfg->frac_bits : 40;
-    if (fb < 16 || fb > 52) {
-        g_create_error = "frac_bits must be in [16,52]";
+    // FP64 mode: 128-bit cells at 2^-60; frac_bits applies to the 64-bit cells of the mixed mode
+    const int fb = cfg->precision == PM3D_FP64 ? kWideFB : (cfg->frac_bits ? cfg->frac_bits : 40);
+    if (fb < 16 || fb > 60) {
+        g_create_error = "frac_bits must be in [16,60]";
         return PM3D_EINVAL;
     }
     Handle* h = new Handle();
@@ -379,9 +386,9 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
     }
     const size_t specN = (size_t)N * N * (N / 2) + (size_t)N * N;
     if ((rc = dalloc(h, &h->x, 3 * h->n)) || (rc = dalloc(h, &h->p, 3 * h->n)) || (rc = dalloc(h, &h->acc, 3 * h->n)) ||
-        (rc = dalloc(h, &h->mesh, h->M)) || (rc = dalloc(h, &h->spec, specN)) || (rc = dalloc(h, &h->gspec, 3 * specN)) ||
+        (rc = dalloc(h, &h->mesh, 2 * h->M)) || (rc = dalloc(h, &h->spec, specN)) || (rc = dalloc(h, &h->gspec, 3 * specN)) ||
         (rc = dalloc(h, &h->grad, 3 * h->M)) || (rc = dalloc(h, &h->tw, (size_t)N)) ||
-        (rc = dalloc(h, &h->dscal, 8 + kRing)) ||
+        (rc = dalloc(h, &h->dscal, 8 + kRing)) || (rc = dalloc(h, &h->bmax, (size_t)N * N)) ||
         (rc = dalloc(h, &h->far, (size_t)h->n)))
         return bail(rc);
     // twiddles exp(-2 pi i k/N) from long-double cos/sin of an exactly reduced angle
@@ -491,13 +498,14 @@ int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho) {
     const size_t xb = 3 * (size_t)n * sizeof(double);
     int rc = ensure_sbuf(h, std::max(xb, (size_t)h->M * sizeof(double)));
     if (rc) return rc;
-    k_fill_u64<<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>(h->mesh, h->M, 0ull);
+    k_fill_u64<<<std::min<long long>(grid_for(2 * h->M), 4096), kThreads, 0, h->st>>>(h->mesh, 2 * h->M, 0ull);
     if (n > 0) {
         HCHK(h, hipMemcpyAsync(h->sbuf, X, xb, hipMemcpyHostToDevice, h->st));
-        k_deposit_atomic<<<grid_for(n), kThreads, 0, h->st>>>(Strided{h->sbuf, 1, 3}, n, h->N, h->dx, h->mesh, h->qs, h->qone);
+        k_deposit_atomic<true><<<grid_for(n), kThreads, 0, h->st>>>(Strided{h->sbuf, 1, 3}, n, h->N, h->dx, h->mesh, h->qs,
+                                                                    h->qone);
     }
-    k_fixed_to_real<<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>((const long long*)h->mesh, h->M,
-                                                                                       h->qscale, h->sbuf);
+    k_fixed_to_real<true><<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 0,
+                                                                                             1.0, h->sbuf);
     HCHK(h, hipMemcpyAsync(rho, h->sbuf, h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
     return check_launch(h, "deposit");
@@ -543,7 +551,9 @@ int pm3d_delta_max_at(void* handle, int64_t serial, double* dmax) {
     HCHK(h, hipMemcpyAsync(&q, h->dscal + 8 + (serial % kRing), sizeof(q), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
     const double nmean = (double)h->Np * h->Np * h->Np / ((double)h->N * h->N * h->N);
-    *dmax = ((double)(long long)q * h->qscale) / nmean - 1.0;
+    double rmax;
+    std::memcpy(&rmax, &q, sizeof(rmax));
+    *dmax = rmax / nmean - 1.0;
     return 0;
 }
 
@@ -554,8 +564,8 @@ int pm3d_get_delta(void* handle, double* delta) {
     int rc = ensure_sbuf(h, h->M * sizeof(double));
     if (rc) return rc;
     const double nmean = (double)h->Np * h->Np * h->Np / ((double)h->N * h->N * h->N);
-    k_fixed_to_delta<<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>((const long long*)h->mesh, h->M,
-                                                                                        h->qscale, nmean, h->sbuf);
+    k_fixed_to_real<true><<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 1,
+                                                                                             nmean, h->sbuf);
     HCHK(h, hipMemcpyAsync(delta, h->sbuf, h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
     return check_launch(h, "get_delta");
