@@ -78,10 +78,25 @@ __device__ __forceinline__ double block_max_nonneg(double v) {
     return __longlong_as_double((long long)m);
 }
 
-// Order-preserving map double -> int64 (for atomicMin on doubles of any sign).
+// Order-preserving map double -> int64 (for min/max of doubles of any sign as integers).
 __device__ __forceinline__ long long ordered_bits(double v) {
     long long i = __double_as_longlong(v);
     return i >= 0 ? i : (i ^ 0x7fffffffffffffffLL);
+}
+__device__ __forceinline__ double from_ordered_bits(long long o) {
+    return __longlong_as_double(o >= 0 ? o : (o ^ 0x7fffffffffffffffLL));
+}
+
+// Block-wide min of a double over kThreads threads (all threads must call).
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ long long wmin[kThreads / 64];
+    const long long o = wave_min_i64(ordered_bits(v));
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = o;
+    __syncthreads();
+    long long m = wmin[0];
+    for (int w = 1; w < kThreads / 64; ++w) m = wmin[w] < m ? wmin[w] : m;
+    __syncthreads();
+    return from_ordered_bits(m);
 }
 
 // ----------------------------------------------------------------- deterministic fixed point

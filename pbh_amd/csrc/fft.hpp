@@ -94,14 +94,13 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R
 // Rows (x,y) of the real mesh -> half spectrum along kz.  FIXED: the input is the fixed-point
 // deposit (WIDE: 128-bit cells), converted on the fly to delta = rho/n_mean - 1 (pm3d.py:103-104),
 // with each block's maximum rho stored to bmax[blockIdx.x] (reduced by the following y pass, so no
-// two blocks contend for one address) and delta optionally written out.  Block 0 also clears the
-// deposit's far-particle counter for the next deposit.
+// two blocks contend for one address).  Block 0 also clears the deposit's far-particle counter for
+// the next deposit.
 template <typename R, int N, bool FIXED, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                     cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
-                                                    R* __restrict__ delta_out, double* __restrict__ bmax,
-                                                    unsigned long long* __restrict__ nfar, double qscale,
-                                                    double nmean) {
+                                                    double* __restrict__ bmax, unsigned long long* __restrict__ nfar,
+                                                    double qscale, double nmean) {
     constexpr int NC = N / 2, P = Plan<R, N>::PZ, PP = P + 1;
     __shared__ cplx<R> A[NC * PP], B[NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
@@ -116,10 +115,6 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
             lmax = fmax(lmax, fmax(r0, r1));
             v0 = (R)(r0 / nmean - 1.0);
             v1 = (R)(r1 / nmean - 1.0);
-            if (delta_out) {
-                delta_out[g] = v0;
-                delta_out[g + 1] = v1;
-            }
         } else {
             v0 = ((const R*)in)[g];
             v1 = ((const R*)in)[g + 1];

@@ -313,10 +313,10 @@ __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ p, const
 
 // ---------------------------------------------------------------------------- shell crossing
 // min over Lagrangian lattice cells of det(dx/dq): forward differences to the +x, +y, +z lattice
-// neighbours, minimum-image unwrapped, divided by the lattice spacing dq.  Folded into *jmin
-// through the order-preserving int64 image of the double.
+// neighbours, minimum-image unwrapped, divided by the lattice spacing dq.  Each block stores its
+// minimum to bmin[blockIdx.x]; the host takes the minimum of those.
 __global__ __launch_bounds__(kThreads) void k_min_jacobian(const double* __restrict__ x, int Np, double L, double dq,
-                                                           long long* __restrict__ jmin) {
+                                                           double* __restrict__ bmin) {
     const long long n = (long long)Np * Np * Np;
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     double det = 1e300;
@@ -338,8 +338,8 @@ __global__ __launch_bounds__(kThreads) void k_min_jacobian(const double* __restr
         det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
     }
-    long long o = wave_min_i64(ordered_bits(det));
-    if ((threadIdx.x & 63) == 0) atomicMin(jmin, o);
+    const double m = block_min(det);
+    if (threadIdx.x == 0) bmin[blockIdx.x] = m;
 }
 
 }  // namespace pm3d

@@ -1,0 +1,281 @@
+// Mixed-precision particle kernels (gfx950).  A position is stored per axis as the CIC base cell
+// and fraction of pm3d.py:76-79 directly:
+//     x = (cell + 1/2 + off) * dx,   cell int32 in [0, N),   off float in [0, 1)
+// so the deposit and the gather need no division and the weights are (1-off, off).  Momenta and
+// accelerations are FP32; the scalars of the step arrive from the host in FP64 and are folded into
+// FP32 factors once per launch.  The deposit mesh is one uint64 per cell at 2^-32: the whole box
+// holds n * 2^32 < 2^63 units for n < 2^31, so it cannot overflow.
+#pragma once
+#include "common.hpp"
+
+namespace pm3d {
+
+constexpr int kNarrowFB = 32;
+
+__device__ __forceinline__ void mx_weights(const float f[3], float w[8]) {
+    const float wx[2] = {1.0f - f[0], f[0]}, wy[2] = {1.0f - f[1], f[1]}, wz[2] = {1.0f - f[2], f[2]};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = (wx[c >> 2] * wy[(c >> 1) & 1]) * wz[c & 1];
+}
+
+// 8 fixed-point weights summing to exactly 2^32 (largest corner takes the remainder).
+__device__ __forceinline__ void mx_quantise8(const float f[3], const float w[8], unsigned long long q[8]) {
+    const int cmax = ((f[0] > 0.5f) << 2) | ((f[1] > 0.5f) << 1) | (f[2] > 0.5f);
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        q[c] = (c == cmax) ? 0ull : (unsigned long long)rintf(w[c] * 4294967296.0f);
+        sum += q[c];
+    }
+    q[cmax] = (1ull << kNarrowFB) - sum;
+}
+
+struct MxPos {
+    const int* cell;
+    const float* off;
+    long long n;
+    __device__ __forceinline__ void load(long long i, int lo[3], float f[3]) const {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = cell[d * n + i];
+            f[d] = off[d * n + i];
+        }
+    }
+};
+
+// Owner-computes brick deposit (see k_deposit_brick in particles.hpp for the scheme).
+template <int BX, int BY, int BZ, int HALO, int NT>
+__global__ __launch_bounds__(NT) void k_deposit_brick_mx(MxPos P, int N, unsigned long long* __restrict__ mesh,
+                                                         int* __restrict__ far, unsigned int* __restrict__ nfar) {
+    constexpr int NCELL = BX * BY * BZ;
+    constexpr int EX = BX + 2 * HALO, EY = BY + 2 * HALO, EZ = BZ + 2 * HALO;
+    __shared__ unsigned long long acc[NCELL];
+    for (int t = threadIdx.x; t < NCELL; t += NT) acc[t] = 0ull;
+    __syncthreads();
+    const int nbz = N / BZ, nby = N / BY;
+    const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
+    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+    for (int t = threadIdx.x; t < EX * EY * EZ; t += NT) {
+        const int ez = t % EZ, ey = (t / EZ) % EY, ex = t / (EZ * EY);
+        const int site[3] = {(x0 + ex - HALO + N) % N, (y0 + ey - HALO + N) % N, (z0 + ez - HALO + N) % N};
+        const long long i = ((long long)site[0] * N + site[1]) * N + site[2];
+        int lo[3];
+        float f[3];
+        P.load(i, lo, f);
+        bool is_far = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int dd = lo[d] - site[d];
+            if (dd > N / 2) dd -= N;
+            if (dd < -N / 2) dd += N;
+            is_far |= (dd < -HALO) || (dd > HALO - 1);
+        }
+        const bool own_site = ex >= HALO && ex < HALO + BX && ey >= HALO && ey < HALO + BY && ez >= HALO && ez < HALO + BZ;
+        const bool app = is_far && own_site;
+        const unsigned long long bal = __ballot(app);
+        if (bal) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)bal) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(nfar, (unsigned int)__popcll(bal));
+            base = __shfl(base, leader, 64);
+            if (app) far[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)i;
+        }
+        if (is_far) continue;
+        float w[8];
+        unsigned long long q[8];
+        mx_weights(f, w);
+        mx_quantise8(f, w, q);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = (c >> 2) ? lo[0] + 1 : lo[0], cy = ((c >> 1) & 1) ? lo[1] + 1 : lo[1], cz = (c & 1) ? lo[2] + 1 : lo[2];
+            const int rx = (cx - x0 + N) % N, ry = (cy - y0 + N) % N, rz = (cz - z0 + N) % N;
+            if (rx < BX && ry < BY && rz < BZ && q[c] != 0) atomicAdd(&acc[(rx * BY + ry) * BZ + rz], q[c]);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NCELL; t += NT) {
+        const int rz = t % BZ, ry = (t / BZ) % BY, rx = t / (BZ * BY);
+        mesh[((size_t)(x0 + rx) * N + (y0 + ry)) * N + (z0 + rz)] = acc[t];
+    }
+}
+
+__device__ __forceinline__ void mx_scatter(const int lo[3], const float f[3], int N, unsigned long long* mesh) {
+    float w[8];
+    unsigned long long q[8];
+    mx_weights(f, w);
+    mx_quantise8(f, w, q);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (q[c] == 0) continue;
+        const int ix = (c >> 2) ? (lo[0] + 1 == N ? 0 : lo[0] + 1) : lo[0];
+        const int iy = ((c >> 1) & 1) ? (lo[1] + 1 == N ? 0 : lo[1] + 1) : lo[1];
+        const int iz = (c & 1) ? (lo[2] + 1 == N ? 0 : lo[2] + 1) : lo[2];
+        atomicAdd(mesh + ((size_t)ix * N + iy) * N + iz, q[c]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_deposit_far_mx(MxPos P, int N, unsigned long long* __restrict__ mesh,
+                                                             const int* __restrict__ far,
+                                                             const unsigned int* __restrict__ nfar) {
+    const unsigned int m = *nfar;
+    for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < m; t += gridDim.x * kThreads) {
+        int lo[3];
+        float f[3];
+        P.load(far[t], lo, f);
+        mx_scatter(lo, f, N, mesh);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_deposit_atomic_mx(MxPos P, int N, unsigned long long* __restrict__ mesh) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= P.n) return;
+    int lo[3];
+    float f[3];
+    P.load(i, lo, f);
+    mx_scatter(lo, f, N, mesh);
+}
+
+// CIC gather of the interleaved FP32 gradient mesh into acc (SoA).
+__global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const float* __restrict__ grad,
+                                                         float* __restrict__ acc) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= P.n) return;
+    int lo[3];
+    float f[3];
+    P.load(i, lo, f);
+    const int hx = lo[0] + 1 == N ? 0 : lo[0] + 1, hy = lo[1] + 1 == N ? 0 : lo[1] + 1, hz = lo[2] + 1 == N ? 0 : lo[2] + 1;
+    const float wx[2] = {1.0f - f[0], f[0]}, wy[2] = {1.0f - f[1], f[1]}, wz[2] = {1.0f - f[2], f[2]};
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int ix = (c >> 2) ? hx : lo[0], iy = ((c >> 1) & 1) ? hy : lo[1], iz = (c & 1) ? hz : lo[2];
+        const float* g = grad + (((size_t)ix * N + iy) * N + iz) * 3;
+        const float w = (wx[c >> 2] * wy[(c >> 1) & 1]) * wz[c & 1];
+        a0 = fmaf(g[0], w, a0);
+        a1 = fmaf(g[1], w, a1);
+        a2 = fmaf(g[2], w, a2);
+    }
+    acc[i] = a0;
+    acc[P.n + i] = a1;
+    acc[2 * P.n + i] = a2;
+}
+
+// Pending closing kick (kp = c1p/denp), opening kick (k1 = c1/den1), drift by p*sd cells
+// (sd = da/(den_d*dx)) with the carry of the offset into the cell index, periodic in [0, N).
+// The cell array is written only where it changes.
+__global__ __launch_bounds__(kThreads) void k_kick_drift_mx(int* __restrict__ cell, float* __restrict__ off,
+                                                            float* __restrict__ p, const float* __restrict__ acc,
+                                                            long long n, int N, int pend, float kp, float k1, float sd,
+                                                            unsigned long long* __restrict__ bad) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    bool ok = true;
+    if (i < n) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const long long k = d * n + i;
+            const float a = acc[k];
+            float pd = p[k];
+            if (pend) pd = fmaf(kp, a, pd);
+            pd = fmaf(k1, a, pd);
+            float f = off[k] + pd * sd;
+            float fl = floorf(f);
+            f -= fl;
+            if (f >= 1.0f) {  // a tiny negative f rounds up to 1 after the subtraction: next cell, offset 0
+                f = 0.0f;
+                fl += 1.0f;
+            }
+            p[k] = pd;
+            off[k] = f;
+            ok &= isfinite(pd) && isfinite(f) && fabsf(fl) < 1e6f;
+            if (fl != 0.0f && ok) {
+                int c = cell[k] + (int)fl;
+                c %= N;
+                if (c < 0) c += N;
+                cell[k] = c;
+            }
+        }
+    }
+    const unsigned long long nb = __ballot(!ok);
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, (unsigned long long)__popcll(nb));
+}
+
+__global__ __launch_bounds__(kThreads) void k_kick_mx(float* __restrict__ p, const float* __restrict__ acc, long long n3,
+                                                      float kp) {
+    const long long k = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (k < n3) p[k] = fmaf(kp, acc[k], p[k]);
+}
+
+// Host FP64 AoS (staged on the device) -> mixed state, and back.
+__global__ __launch_bounds__(kThreads) void k_set_pos_mx(const double* __restrict__ xa, int* __restrict__ cell,
+                                                         float* __restrict__ off, long long n, int N, double dx) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int lo, hi;
+        double f;
+        cic_axis(xa[i * 3 + d], dx, N, lo, hi, f);
+        float ff = (float)f;
+        if (ff >= 1.0f) {
+            ff = 0.0f;
+            lo = hi;
+        }
+        cell[d * n + i] = lo;
+        off[d * n + i] = ff;
+    }
+}
+__global__ __launch_bounds__(kThreads) void k_get_pos_mx(const int* __restrict__ cell, const float* __restrict__ off,
+                                                         double* __restrict__ xa, long long n, double dx, double L) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double v = ((double)cell[d * n + i] + 0.5 + (double)off[d * n + i]) * dx;
+        if (v >= L) v -= L;
+        xa[i * 3 + d] = v;
+    }
+}
+__global__ __launch_bounds__(kThreads) void k_f64_to_f32_soa(const double* __restrict__ a, float* __restrict__ s, long long n) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d * n + i] = (float)a[i * 3 + d];
+}
+__global__ __launch_bounds__(kThreads) void k_f32_soa_to_f64(const float* __restrict__ s, double* __restrict__ a, long long n) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[i * 3 + d] = (double)s[d * n + i];
+}
+
+// min det(dx/dq) over the Lagrangian lattice from cell + offset (differences exact in cells).
+__global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, int N, double cells_per_site,
+                                                              double* __restrict__ bmin) {
+    const long long n = P.n;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    double det = 1e300;
+    if (i < n) {
+        const int a = (int)(i / ((long long)Np * Np)), b = (int)((i / Np) % Np), c = (int)(i % Np);
+        const long long nb[3] = {((long long)((a + 1) % Np) * Np + b) * Np + c,
+                                 ((long long)a * Np + (b + 1) % Np) * Np + c,
+                                 ((long long)a * Np + b) * Np + (c + 1) % Np};
+        double J[3][3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                int dc = P.cell[d * n + nb[e]] - P.cell[d * n + i];
+                if (dc > N / 2) dc -= N;
+                if (dc < -N / 2) dc += N;
+                J[e][d] = ((double)dc + ((double)P.off[d * n + nb[e]] - (double)P.off[d * n + i])) / cells_per_site;
+            }
+        }
+        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    }
+    const double m = block_min(det);
+    if (threadIdx.x == 0) bmin[blockIdx.x] = m;
+}
+
+}  // namespace pm3d

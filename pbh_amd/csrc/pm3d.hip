@@ -1,15 +1,18 @@
 // pm3d particle-mesh library for MI355X (gfx950): C-ABI entry points of include/pm3d_hip.h.
 //
 // One handle = one periodic box stepped on one HIP stream.  Device state is SoA in Lagrangian
-// particle order (particle i is lattice site i, pm3d.py:64-65), so the x/y/z components of the
-// positions, momenta and cached accelerations are each n contiguous doubles.
+// particle order (particle i is lattice site i, pm3d.py:64-65).
+//   FP64 mode : x, p, acc as 3 x n doubles; deposit mesh 128-bit fixed point {lo,hi} per cell.
+//   mixed mode: cell (3 x n int32) + off (3 x n float), p and acc 3 x n float; deposit mesh one
+//               uint64 per cell at 2^-32; FFT, Green's function and gradient mesh in FP32.
 //
-// A step (pm3d.py:117-129) is 8 launches:
-//   kick+drift -> fill(mesh) -> deposit -> r2c_z (int64 -> delta fused) -> y fwd
+// A step (pm3d.py:117-129) is:
+//   kick+drift -> deposit (+ far pass) -> r2c_z (fixed point -> delta fused) -> y fwd
 //   -> x (fwd FFT, Green's function, -ik, 3 inverse FFTs) -> y inv (3 comps) -> c2r_z (3 comps)
-//   -> gather (+ closing kick)
+//   -> gather (stores acc; the closing kick is applied with the next opening kick)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +23,7 @@
 #include "common.hpp"
 #include "fft.hpp"
 #include "particles.hpp"
+#include "particles_mixed.hpp"
 
 using namespace pm3d;
 
@@ -42,41 +46,41 @@ enum KernelClass {
     KC_OTHER,
     KC_COUNT
 };
-const char* kKernelNames =
-    "kick_drift,fill,deposit,deposit_far,r2c_z,y_fwd,x_poisson,y_inv,c2r_z,gather,other";
+const char* kKernelNames = "kick_drift,fill,deposit,deposit_far,r2c_z,y_fwd,x_poisson,y_inv,c2r_z,gather,other";
 
 struct Handle {
     pm3d_cfg cfg{};
     int N = 0, Np = 0, log2N = 0;
     long long n = 0, M = 0;
-    double L = 0, dx = 0, qscale = 0, qs = 0;
+    double L = 0, dx = 0, qscale = 0, qs = 0, nmean = 1;
     long long qone = 0;
+    bool mixed = false;
+    bool wide = true;  // 128-bit mesh cells (FP64 mode)
     hipStream_t st = nullptr;
-    // particle state (FP64 SoA)
+    // particle state, FP64 mode
     double *x = nullptr, *p = nullptr, *acc = nullptr;
+    // particle state, mixed mode
+    int* cell = nullptr;
+    float *off = nullptr, *pf = nullptr, *accf = nullptr;
     // meshes
-    unsigned long long* mesh = nullptr;  // fixed-point deposit: FP64 mode {lo,hi} per cell (2^-60)
-    bool wide = true;
-    cplx<double>* spec = nullptr;        // [N][N][N/2] + nyq [N][N]
-    cplx<double>* gspec = nullptr;       // 3 x the same
-    double* grad = nullptr;              // [M][3]
-    double* delta = nullptr;             // [M] (written on demand)
-    cplx<double>* tw = nullptr;          // exp(-2 pi i k/N), k < N
-    // device scalars: [0] nfar, [1] bad count (sticky until pm3d_synchronize), [2] jmin bits,
-    // [8 + s % kRing] max fixed-point cell value of deposit serial s
+    unsigned long long* mesh = nullptr;
+    void* spec = nullptr;   // [N][N][N/2] + nyq [N][N], complex R
+    void* gspec = nullptr;  // 3 x the same
+    void* grad = nullptr;   // [M][3] R
+    void* tw = nullptr;     // exp(-2 pi i k/N), complex R
+    // device scalars: [0] nfar, [1] bad count (sticky until pm3d_synchronize), [2..7] unused,
+    // [8 + s % kRing] max rho (double bits) of deposit serial s
     unsigned long long* dscal = nullptr;
-    long long serial = 0;                // number of deposits of the state so far
-    int* far = nullptr;                  // far-particle list of the tiled deposit
-    double* bmax = nullptr;              // per-block maxima of the r2c pass
-    // staging for reference-shaped entry points
-    double* sbuf = nullptr;
+    long long serial = 0;
+    int* far = nullptr;
+    double* bmax = nullptr;  // per-block maxima / minima scratch
+    size_t bmax_len = 0;
+    double* sbuf = nullptr;  // staging for reference-shaped entry points
     size_t sbuf_bytes = 0;
     bool state_set = false;
     bool acc_valid = false;
-    // deferred closing kick (pm3d.py:127) -- applied before the next kick or a read of p
     bool kick_pending = false;
     double pend_c1 = 0, pend_den = 0;
-    int deposit_mode = 1;  // 0 atomic, 1 tiled
     // timing
     bool timing = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
@@ -87,13 +91,13 @@ struct Handle {
     size_t bytes = 0;
 };
 
-#define HCHK(h, call)                                                                         \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess) {                                                               \
-            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);                     \
-            return PM3D_EHIP;                                                                 \
-        }                                                                                     \
+#define HCHK(h, call)                                                     \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) {                                           \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_); \
+            return PM3D_EHIP;                                             \
+        }                                                                 \
     } while (0)
 
 int fail(Handle* h, int code, const std::string& msg) {
@@ -115,110 +119,141 @@ hipEvent_t take_event(Handle* h) {
 // Brackets one launch with events when timing is on.
 struct Timed {
     Handle* h;
-    int kc;
     hipEvent_t e1 = nullptr;
-    Timed(Handle* h_, int kc_) : h(h_), kc(kc_) {
+    Timed(Handle* h_, int kc) : h(h_) {
         if (h->timing) {
             hipEvent_t e0 = take_event(h);
             e1 = take_event(h);
             if (e0 && e1) {
                 hipEventRecord(e0, h->st);
                 h->ev_used.push_back({kc, {e0, e1}});
+            } else {
+                e1 = nullptr;
             }
         }
     }
     ~Timed() {
-        if (h->timing && e1) hipEventRecord(e1, h->st);
+        if (e1) hipEventRecord(e1, h->st);
     }
 };
 
 inline unsigned grid_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+inline unsigned grid_cap(long long n) { return (unsigned)std::min<long long>(grid_for(n), 4096); }
 inline unsigned long long* qmax_slot(Handle* h) { return h->dscal + 8 + (h->serial % kRing); }
-
-// ---------------------------------------------------------------- dispatch over power-of-two N
-template <int N>
-int launch_fft_forward_tmpl(Handle* h, const void* in, bool fixed, double* delta_out, double nmean) {
-    using R = double;
-    const size_t NH = N / 2, specN = (size_t)N * N * NH;
-    cplx<R>* spec = h->spec;
-    cplx<R>* nyq = h->spec + specN;
-    {
-        Timed t(h, KC_R2C_Z);
-        const unsigned g = (unsigned)((size_t)N * N / Plan<R, N>::PZ);
-        if (fixed)
-            k_r2c_z<R, N, true, true><<<g, kThreads, 0, h->st>>>(in, spec, nyq, h->tw, delta_out, h->bmax, h->dscal,
-                                                                 h->qscale, nmean);
-        else
-            k_r2c_z<R, N, false, true><<<g, kThreads, 0, h->st>>>(in, spec, nyq, h->tw, nullptr, nullptr, nullptr, 0.0,
-                                                                  1.0);
-    }
-    {
-        Timed t(h, KC_Y_FWD);
-        constexpr int P = Plan<R, N>::PY;
-        dim3 g(N * (N / 2 / P) + N / P, 1);
-        const int nb = fixed ? (int)((size_t)N * N / Plan<R, N>::PZ) : 0;
-        k_c2c_y<R, N, false><<<g, kThreads, 0, h->st>>>(spec, nyq, 0, 0, h->tw, h->bmax, nb, qmax_slot(h));
-    }
-    return 0;
-}
-
-template <int N>
-int launch_x_tmpl(Handle* h, int mode, double s, double sm2) {
-    using R = double;
-    const size_t NH = N / 2, specN = (size_t)N * N * NH, tot = specN + (size_t)N * N;
-    constexpr int P = Plan<R, N>::PX;
-    const unsigned g = N * (N / 2 / P) + N / P;
-    const double kval = 1.0 / ((double)N * h->dx);
-    Timed t(h, KC_X_POISSON);
-    if (mode == 0)
-        k_x_pass<R, N, 0><<<g, kThreads, 0, h->st>>>(h->spec, h->spec + specN, nullptr, nullptr, 0, 0, h->tw, kval, 0, 0, 0);
-    else if (mode == 1)
-        k_x_pass<R, N, 1><<<g, kThreads, 0, h->st>>>(h->spec, h->spec + specN, nullptr, nullptr, 0, 0, h->tw, kval, 0, 0, 0);
-    else
-        k_x_pass<R, N, 2><<<g, kThreads, 0, h->st>>>(h->spec, h->spec + specN, h->gspec, h->gspec + specN, tot, tot, h->tw,
-                                                     kval, s, sm2, sm2 > 0.0 ? 1 : 0);
-    return 0;
-}
-
-// Inverse y (ncomp spectra starting at base) and c2r z into out (ncomp interleaved).
-template <int N>
-int launch_inverse_tmpl(Handle* h, cplx<double>* base, int ncomp, double* out) {
-    using R = double;
-    const size_t NH = N / 2, specN = (size_t)N * N * NH, tot = specN + (size_t)N * N;
-    {
-        Timed t(h, KC_Y_INV);
-        constexpr int P = Plan<R, N>::PY;
-        dim3 g(N * (N / 2 / P) + N / P, ncomp);
-        k_c2c_y<R, N, true><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, h->tw, nullptr, 0, nullptr);
-    }
-    {
-        Timed t(h, KC_C2R_Z);
-        const unsigned g = (unsigned)((size_t)N * N / Plan<R, N>::PC);
-        const double scale = 1.0 / ((double)N * N * N);
-        if (ncomp == 3)
-            k_c2r_z<R, N, 3><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, out, h->tw, scale);
-        else
-            k_c2r_z<R, N, 1><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, out, h->tw, scale);
-    }
-    return 0;
-}
-
-#define PM3D_DISPATCH(h, FN, ...)                    \
-    switch ((h)->log2N) {                            \
-        case 3: FN<8>(h, __VA_ARGS__); break;        \
-        case 4: FN<16>(h, __VA_ARGS__); break;       \
-        case 5: FN<32>(h, __VA_ARGS__); break;       \
-        case 6: FN<64>(h, __VA_ARGS__); break;       \
-        case 7: FN<128>(h, __VA_ARGS__); break;      \
-        case 8: FN<256>(h, __VA_ARGS__); break;      \
-        case 9: FN<512>(h, __VA_ARGS__); break;      \
-        default: return fail(h, PM3D_EUNSUPPORTED, "N must be a power of two in [8,512]"); \
-    }
+inline MxPos mxpos(Handle* h) { return MxPos{h->cell, h->off, h->n}; }
 
 int check_launch(Handle* h, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(h, PM3D_EHIP, std::string(what) + ": " + hipGetErrorString(e));
     return 0;
+}
+
+// ---------------------------------------------------------------- FFT launches, R and N templated
+template <typename R, int N>
+struct Fft {
+    static constexpr size_t NH = N / 2, specN = (size_t)N * N * NH, tot = specN + (size_t)N * N;
+
+    // r2c z + forward y on h->spec.  fixed: input is the deposit mesh.
+    static void forward(Handle* h, const void* in, bool fixed) {
+        cplx<R>* spec = (cplx<R>*)h->spec;
+        cplx<R>* nyq = spec + specN;
+        const cplx<R>* tw = (const cplx<R>*)h->tw;
+        const unsigned gz = (unsigned)((size_t)N * N / Plan<R, N>::PZ);
+        {
+            Timed t(h, KC_R2C_Z);
+            if (fixed && h->wide)
+                k_r2c_z<R, N, true, true><<<gz, kThreads, 0, h->st>>>(in, spec, nyq, tw, h->bmax, h->dscal, h->qscale, h->nmean);
+            else if (fixed)
+                k_r2c_z<R, N, true, false><<<gz, kThreads, 0, h->st>>>(in, spec, nyq, tw, h->bmax, h->dscal, h->qscale, h->nmean);
+            else
+                k_r2c_z<R, N, false, false><<<gz, kThreads, 0, h->st>>>(in, spec, nyq, tw, nullptr, nullptr, 0.0, 1.0);
+        }
+        {
+            Timed t(h, KC_Y_FWD);
+            constexpr int P = Plan<R, N>::PY;
+            dim3 g(N * (N / 2 / P) + N / P, 1);
+            k_c2c_y<R, N, false><<<g, kThreads, 0, h->st>>>(spec, nyq, 0, 0, tw, h->bmax, fixed ? (int)gz : 0, qmax_slot(h));
+        }
+    }
+
+    // x pass: mode 0 forward, 1 inverse (in place on h->spec), 2 Poisson (h->spec -> h->gspec)
+    static void xpass(Handle* h, int mode, double s, double sm2) {
+        cplx<R>* spec = (cplx<R>*)h->spec;
+        cplx<R>* gs = (cplx<R>*)h->gspec;
+        const cplx<R>* tw = (const cplx<R>*)h->tw;
+        constexpr int P = Plan<R, N>::PX;
+        const unsigned g = N * (N / 2 / P) + N / P;
+        const double kval = 1.0 / ((double)N * h->dx);
+        Timed t(h, KC_X_POISSON);
+        if (mode == 0)
+            k_x_pass<R, N, 0><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0);
+        else if (mode == 1)
+            k_x_pass<R, N, 1><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0);
+        else
+            k_x_pass<R, N, 2><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s, sm2,
+                                                         sm2 > 0.0 ? 1 : 0);
+    }
+
+    // inverse y (ncomp spectra from base) + c2r z into out (ncomp interleaved)
+    static void inverse(Handle* h, void* base_, int ncomp, void* out) {
+        cplx<R>* base = (cplx<R>*)base_;
+        const cplx<R>* tw = (const cplx<R>*)h->tw;
+        {
+            Timed t(h, KC_Y_INV);
+            constexpr int P = Plan<R, N>::PY;
+            dim3 g(N * (N / 2 / P) + N / P, ncomp);
+            k_c2c_y<R, N, true><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, tw, nullptr, 0, nullptr);
+        }
+        {
+            Timed t(h, KC_C2R_Z);
+            const unsigned g = (unsigned)((size_t)N * N / Plan<R, N>::PC);
+            const R scale = (R)(1.0 / ((double)N * N * N));
+            if (ncomp == 3)
+                k_c2r_z<R, N, 3><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+            else
+                k_c2r_z<R, N, 1><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+        }
+    }
+};
+
+template <typename R>
+int fft_call(Handle* h, int op, const void* in, bool fixed, int mode, double s, double sm2, void* base, int ncomp,
+             void* out) {
+#define PM3D_FFT_CASE(NN)                                     \
+    case NN:                                                  \
+        if (op == 0)                                          \
+            Fft<R, NN>::forward(h, in, fixed);                \
+        else if (op == 1)                                     \
+            Fft<R, NN>::xpass(h, mode, s, sm2);               \
+        else                                                  \
+            Fft<R, NN>::inverse(h, base, ncomp, out);         \
+        break;
+    switch (h->N) {
+        PM3D_FFT_CASE(8)
+        PM3D_FFT_CASE(16)
+        PM3D_FFT_CASE(32)
+        PM3D_FFT_CASE(64)
+        PM3D_FFT_CASE(128)
+        PM3D_FFT_CASE(256)
+        PM3D_FFT_CASE(512)
+        default:
+            return fail(h, PM3D_EUNSUPPORTED, "N must be a power of two in [8,512]");
+    }
+#undef PM3D_FFT_CASE
+    return 0;
+}
+
+int fft_forward(Handle* h, const void* in, bool fixed) {
+    return h->mixed ? fft_call<float>(h, 0, in, fixed, 0, 0, 0, nullptr, 0, nullptr)
+                    : fft_call<double>(h, 0, in, fixed, 0, 0, 0, nullptr, 0, nullptr);
+}
+int fft_xpass(Handle* h, int mode, double s, double sm2) {
+    return h->mixed ? fft_call<float>(h, 1, nullptr, false, mode, s, sm2, nullptr, 0, nullptr)
+                    : fft_call<double>(h, 1, nullptr, false, mode, s, sm2, nullptr, 0, nullptr);
+}
+int fft_inverse(Handle* h, void* base, int ncomp, void* out) {
+    return h->mixed ? fft_call<float>(h, 2, nullptr, false, 0, 0, 0, base, ncomp, out)
+                    : fft_call<double>(h, 2, nullptr, false, 0, 0, 0, base, ncomp, out);
 }
 
 int ensure_sbuf(Handle* h, size_t bytes) {
@@ -232,50 +267,63 @@ int ensure_sbuf(Handle* h, size_t bytes) {
     return 0;
 }
 
-// Deposit of the device SoA state into h->mesh (int64 fixed point).  The tiled path writes every
-// cell, so only the atomic path clears the mesh first.
+// Deposit of the device state into h->mesh.  The brick path writes every cell; only the atomic
+// path (Np != N or N < 32) clears the mesh first.  nfar is cleared by the r2c pass of the previous
+// deposit (and at creation); this deposit's maximum is stored into its ring slot by the y pass.
 int deposit_state(Handle* h) {
-    // bricks of 16^3 cells with a one-site halo: the halo must not wrap onto the brick itself
-    const bool tiled = h->deposit_mode == 1 && h->Np == h->N && h->N >= 32;
-    // nfar is cleared by the r2c pass of the previous deposit (and at creation); the maximum of
-    // this deposit is stored (not accumulated) into its ring slot by the y pass
+    const bool tiled = h->Np == h->N && h->N >= 32;  // 16^3 bricks with a 1-site halo must not wrap
     h->serial += 1;
+    const int W = h->wide ? 2 : 1;
     if (!tiled) {
         Timed t(h, KC_FILL);
-        k_fill_u64<<<std::min<long long>(grid_for(2 * h->M), 4096), kThreads, 0, h->st>>>(h->mesh, 2 * h->M, 0ull);
+        k_fill_u64<<<grid_cap(W * h->M), kThreads, 0, h->st>>>(h->mesh, W * h->M, 0ull);
     }
-    if (tiled) {
-        {
-            Timed t(h, KC_DEPOSIT);
-            const int B = 16;
-            const unsigned g = (unsigned)((h->N / B) * (h->N / B) * (h->N / B));
-            k_deposit_brick<16, 16, 16, 1, true, 1024><<<g, 1024, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs,
-                                                                               h->qone, h->far, (unsigned int*)(h->dscal + 0));
-        }
-        {
+    unsigned int* nfar = (unsigned int*)(h->dscal + 0);
+    const unsigned gb = (unsigned)((h->N / 16) * (h->N / 16) * (h->N / 16));
+    if (!h->mixed) {
+        if (tiled) {
+            {
+                Timed t(h, KC_DEPOSIT);
+                k_deposit_brick<16, 16, 16, 1, true, 1024>
+                    <<<gb, 1024, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone, h->far, nfar);
+            }
             Timed t(h, KC_DEPOSIT_FAR);
-            k_deposit_far<true><<<1024, kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone, h->far,
-                                                        (const unsigned int*)(h->dscal + 0));
+            k_deposit_far<true><<<1024, kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone, h->far, nfar);
+        } else {
+            Timed t(h, KC_DEPOSIT);
+            k_deposit_atomic<true>
+                <<<grid_for(h->n), kThreads, 0, h->st>>>(Strided{h->x, h->n, 1}, h->n, h->N, h->dx, h->mesh, h->qs, h->qone);
         }
     } else {
-        Timed t(h, KC_DEPOSIT);
-        Strided X{h->x, h->n, 1};
-        k_deposit_atomic<true><<<grid_for(h->n), kThreads, 0, h->st>>>(X, h->n, h->N, h->dx, h->mesh, h->qs, h->qone);
+        if (tiled) {
+            {
+                Timed t(h, KC_DEPOSIT);
+                k_deposit_brick_mx<16, 16, 16, 1, 1024><<<gb, 1024, 0, h->st>>>(mxpos(h), h->N, h->mesh, h->far, nfar);
+            }
+            Timed t(h, KC_DEPOSIT_FAR);
+            k_deposit_far_mx<<<1024, kThreads, 0, h->st>>>(mxpos(h), h->N, h->mesh, h->far, nfar);
+        } else {
+            Timed t(h, KC_DEPOSIT);
+            k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, h->mesh);
+        }
     }
     return check_launch(h, "deposit");
 }
 
-// forces at the current positions: deposit, Poisson, gather into acc_out (SoA, 3n).
-int forces_state(Handle* h, double s, double sm2, double* delta_out, double* acc_out) {
-    int rc = deposit_state(h);
-    if (rc) return rc;
-    const double nmean = (double)h->Np * h->Np * h->Np / ((double)h->N * h->N * h->N);
-    PM3D_DISPATCH(h, launch_fft_forward_tmpl, (const void*)h->mesh, true, delta_out, nmean);
-    PM3D_DISPATCH(h, launch_x_tmpl, 2, s, sm2);
-    PM3D_DISPATCH(h, launch_inverse_tmpl, h->gspec, 3, h->grad);
+// forces at the current positions: deposit, Poisson, gather into acc_out (SoA 3n, state precision).
+int forces_state(Handle* h, double s, double sm2, void* acc_out) {
+    int rc;
+    if ((rc = deposit_state(h))) return rc;
+    if ((rc = fft_forward(h, h->mesh, true))) return rc;
+    if ((rc = fft_xpass(h, 2, s, sm2))) return rc;
+    if ((rc = fft_inverse(h, h->gspec, 3, h->grad))) return rc;
     {
         Timed t(h, KC_GATHER);
-        k_gather3<false><<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->grad, acc_out, h->p, 0, 1);
+        if (!h->mixed)
+            k_gather3<false><<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, (const double*)h->grad,
+                                                                     (double*)acc_out, h->p, 0, 1);
+        else
+            k_gather3_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, (const float*)h->grad, (float*)acc_out);
     }
     return check_launch(h, "forces");
 }
@@ -283,7 +331,10 @@ int forces_state(Handle* h, double s, double sm2, double* delta_out, double* acc
 int flush_kick(Handle* h) {
     if (!h->kick_pending) return 0;
     Timed t(h, KC_OTHER);
-    k_kick<<<grid_for(3 * h->n), kThreads, 0, h->st>>>(h->p, h->acc, 3 * h->n, h->pend_c1, h->pend_den);
+    if (!h->mixed)
+        k_kick<<<grid_for(3 * h->n), kThreads, 0, h->st>>>(h->p, h->acc, 3 * h->n, h->pend_c1, h->pend_den);
+    else
+        k_kick_mx<<<grid_for(3 * h->n), kThreads, 0, h->st>>>(h->pf, h->accf, 3 * h->n, (float)(h->pend_c1 / h->pend_den));
     h->kick_pending = false;
     return check_launch(h, "kick");
 }
@@ -298,16 +349,10 @@ int read_bad(Handle* h, int* bad) {
 }
 
 void free_all(Handle* h) {
-    double* bufs[] = {h->x, h->p, h->acc, h->grad, h->delta, h->sbuf};
-    for (double* b : bufs)
+    void* bufs[] = {h->x, h->p, h->acc, h->cell, h->off, h->pf, h->accf, h->mesh, h->spec,
+                    h->gspec, h->grad, h->tw, h->dscal, h->far, h->bmax, h->sbuf};
+    for (void* b : bufs)
         if (b) hipFree(b);
-    if (h->mesh) hipFree(h->mesh);
-    if (h->spec) hipFree(h->spec);
-    if (h->gspec) hipFree(h->gspec);
-    if (h->tw) hipFree(h->tw);
-    if (h->dscal) hipFree(h->dscal);
-    if (h->far) hipFree(h->far);
-    if (h->bmax) hipFree(h->bmax);
     for (auto& u : h->ev_used) {
         hipEventDestroy(u.second.first);
         hipEventDestroy(u.second.second);
@@ -325,6 +370,90 @@ int dalloc(Handle* h, T** ptr, size_t count) {
         return PM3D_ENOMEM;
     }
     h->bytes += b;
+    return 0;
+}
+int dalloc_bytes(Handle* h, void** ptr, size_t b) { return dalloc(h, (char**)ptr, b); }
+
+// delta = rho/n_mean - 1 of the most recent deposit into FP64 device memory `out`.
+void mesh_to_delta(Handle* h, double* out) {
+    if (h->wide)
+        k_fixed_to_real<true><<<grid_cap(h->M), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 1, h->nmean, out);
+    else
+        k_fixed_to_real<false><<<grid_cap(h->M), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 1, h->nmean, out);
+}
+
+// numpy-layout half spectrum (N,N,N/2+1) <-> device layout
+template <typename R>
+void spec_to_numpy(const std::vector<cplx<R>>& hs, int N, double* out_c) {
+    const size_t NH = N / 2, specN = (size_t)N * N * NH;
+    for (size_t r = 0; r < (size_t)N * N; ++r) {
+        for (size_t k = 0; k < NH; ++k) {
+            out_c[2 * (r * (NH + 1) + k)] = hs[r * NH + k].re;
+            out_c[2 * (r * (NH + 1) + k) + 1] = hs[r * NH + k].im;
+        }
+        out_c[2 * (r * (NH + 1) + NH)] = hs[specN + r].re;
+        out_c[2 * (r * (NH + 1) + NH) + 1] = hs[specN + r].im;
+    }
+}
+template <typename R>
+void numpy_to_spec(const double* in_c, int N, std::vector<cplx<R>>& hs) {
+    const size_t NH = N / 2, specN = (size_t)N * N * NH;
+    hs.resize(specN + (size_t)N * N);
+    for (size_t r = 0; r < (size_t)N * N; ++r) {
+        for (size_t k = 0; k < NH; ++k)
+            hs[r * NH + k] = cplx<R>{(R)in_c[2 * (r * (NH + 1) + k)], (R)in_c[2 * (r * (NH + 1) + k) + 1]};
+        hs[specN + r] = cplx<R>{(R)in_c[2 * (r * (NH + 1) + NH)], (R)in_c[2 * (r * (NH + 1) + NH) + 1]};
+    }
+}
+
+template <typename R>
+int rfftn_impl(Handle* h, const double* in, double* out_c) {
+    std::vector<R> hin(h->M);
+    for (long long i = 0; i < h->M; ++i) hin[i] = (R)in[i];
+    int rc = ensure_sbuf(h, h->M * sizeof(double));
+    if (rc) return rc;
+    HCHK(h, hipMemcpyAsync(h->sbuf, hin.data(), h->M * sizeof(R), hipMemcpyHostToDevice, h->st));
+    if ((rc = fft_forward(h, h->sbuf, false)) || (rc = fft_xpass(h, 0, 0, 0))) return rc;
+    if ((rc = check_launch(h, "rfftn"))) return rc;
+    const size_t specN = (size_t)h->N * h->N * (h->N / 2);
+    std::vector<cplx<R>> hs(specN + (size_t)h->N * h->N);
+    HCHK(h, hipMemcpyAsync(hs.data(), h->spec, hs.size() * sizeof(cplx<R>), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipStreamSynchronize(h->st));
+    spec_to_numpy(hs, h->N, out_c);
+    return 0;
+}
+
+template <typename R>
+int irfftn_impl(Handle* h, const double* in_c, double* out) {
+    std::vector<cplx<R>> hs;
+    numpy_to_spec(in_c, h->N, hs);
+    int rc = ensure_sbuf(h, h->M * sizeof(double));
+    if (rc) return rc;
+    HCHK(h, hipMemcpyAsync(h->spec, hs.data(), hs.size() * sizeof(cplx<R>), hipMemcpyHostToDevice, h->st));
+    if ((rc = fft_xpass(h, 1, 0, 0)) || (rc = fft_inverse(h, h->spec, 1, h->sbuf))) return rc;
+    if ((rc = check_launch(h, "irfftn"))) return rc;
+    std::vector<R> ho(h->M);
+    HCHK(h, hipMemcpyAsync(ho.data(), h->sbuf, h->M * sizeof(R), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipStreamSynchronize(h->st));
+    for (long long i = 0; i < h->M; ++i) out[i] = (double)ho[i];
+    return 0;
+}
+
+template <typename R>
+int poisson_impl(Handle* h, const double* delta, double s, double sm2, double* g) {
+    std::vector<R> hin(h->M);
+    for (long long i = 0; i < h->M; ++i) hin[i] = (R)delta[i];
+    int rc = ensure_sbuf(h, h->M * sizeof(double));
+    if (rc) return rc;
+    HCHK(h, hipMemcpyAsync(h->sbuf, hin.data(), h->M * sizeof(R), hipMemcpyHostToDevice, h->st));
+    if ((rc = fft_forward(h, h->sbuf, false)) || (rc = fft_xpass(h, 2, s, sm2)) ||
+        (rc = fft_inverse(h, h->gspec, 3, h->grad)))
+        return rc;
+    if ((rc = check_launch(h, "poisson_gradient"))) return rc;
+    std::vector<R> ho(3 * h->M);
+    HCHK(h, hipMemcpyAsync(ho.data(), h->grad, 3 * h->M * sizeof(R), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipStreamSynchronize(h->st));
+    for (long long i = 0; i < 3 * h->M; ++i) g[i] = (double)ho[i];
     return 0;
 }
 
@@ -346,30 +475,34 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         g_create_error = "N must be a power of two in [8,512]";
         return PM3D_EUNSUPPORTED;
     }
-    if (cfg->Np < 1 || !(cfg->L > 0.0) || cfg->smooth < 0.0) {
+    if (cfg->Np < 1 || cfg->Np > 1024 || !(cfg->L > 0.0) || !(cfg->smooth >= 0.0)) {
         g_create_error = "invalid Np, L or smooth";
         return PM3D_EINVAL;
     }
-    if (cfg->precision != PM3D_FP64) {
-        g_create_error = "precision mode not built";
-        return PM3D_EUNSUPPORTED;
+    if (cfg->precision != PM3D_FP64 && cfg->precision != PM3D_MIXED) {
+        g_create_error = "unknown precision mode";
+        return PM3D_EINVAL;
     }
-    // FP64 mode: 128-bit cells at 2^-60; frac_bits applies to the 64-bit cells of the mixed mode
-    const int fb = cfg->precision == PM3D_FP64 ? kWideFB : (cfg->frac_bits ? cfg->frac_bits : 40);
-    if (fb < 16 || fb > 60) {
-        g_create_error = "frac_bits must be in [16,60]";
+    const bool mixed = cfg->precision == PM3D_MIXED;
+    // FP64 mode: 128-bit cells at 2^-60.  Mixed mode: 64-bit cells at 2^-32 (cannot overflow).
+    const int fb = mixed ? kNarrowFB : kWideFB;
+    if (cfg->frac_bits != 0 && cfg->frac_bits != fb) {
+        g_create_error = "frac_bits is fixed by the precision mode (FP64: 60 on 128-bit cells, mixed: 32); pass 0";
         return PM3D_EINVAL;
     }
     Handle* h = new Handle();
     h->cfg = *cfg;
     h->cfg.frac_bits = fb;
+    h->mixed = mixed;
+    h->wide = !mixed;
     h->N = N;
     h->log2N = lg;
     h->Np = cfg->Np;
     h->n = (long long)cfg->Np * cfg->Np * cfg->Np;
     h->M = (long long)N * N * N;
     h->L = cfg->L;
-    h->dx = cfg->L / N;  // pm3d.py:30
+    h->dx = cfg->L / N;                      // pm3d.py:30
+    h->nmean = (double)h->n / (double)h->M;  // pm3d.py:103, Np**3/N**3
     h->qs = std::ldexp(1.0, fb);
     h->qscale = std::ldexp(1.0, -fb);
     h->qone = 1LL << fb;
@@ -385,21 +518,34 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         return bail(PM3D_EHIP);
     }
     const size_t specN = (size_t)N * N * (N / 2) + (size_t)N * N;
-    if ((rc = dalloc(h, &h->x, 3 * h->n)) || (rc = dalloc(h, &h->p, 3 * h->n)) || (rc = dalloc(h, &h->acc, 3 * h->n)) ||
-        (rc = dalloc(h, &h->mesh, 2 * h->M)) || (rc = dalloc(h, &h->spec, specN)) || (rc = dalloc(h, &h->gspec, 3 * specN)) ||
-        (rc = dalloc(h, &h->grad, 3 * h->M)) || (rc = dalloc(h, &h->tw, (size_t)N)) ||
-        (rc = dalloc(h, &h->dscal, 8 + kRing)) || (rc = dalloc(h, &h->bmax, (size_t)N * N)) ||
-        (rc = dalloc(h, &h->far, (size_t)h->n)))
+    const size_t rb = mixed ? 4 : 8;  // bytes per real
+    const long long n3 = 3 * h->n;
+    if (!mixed) {
+        if ((rc = dalloc(h, &h->x, n3)) || (rc = dalloc(h, &h->p, n3)) || (rc = dalloc(h, &h->acc, n3))) return bail(rc);
+    } else {
+        if ((rc = dalloc(h, &h->cell, n3)) || (rc = dalloc(h, &h->off, n3)) || (rc = dalloc(h, &h->pf, n3)) ||
+            (rc = dalloc(h, &h->accf, n3)))
+            return bail(rc);
+    }
+    h->bmax_len = std::max<size_t>((size_t)N * N, (size_t)grid_for(h->n));
+    if ((rc = dalloc(h, &h->mesh, (h->wide ? 2 : 1) * h->M)) || (rc = dalloc_bytes(h, &h->spec, specN * 2 * rb)) ||
+        (rc = dalloc_bytes(h, &h->gspec, 3 * specN * 2 * rb)) || (rc = dalloc_bytes(h, &h->grad, 3 * h->M * rb)) ||
+        (rc = dalloc_bytes(h, &h->tw, (size_t)N * 2 * rb)) || (rc = dalloc(h, &h->dscal, 8 + kRing)) ||
+        (rc = dalloc(h, &h->bmax, h->bmax_len)) || (rc = dalloc(h, &h->far, (size_t)h->n)))
         return bail(rc);
-    // twiddles exp(-2 pi i k/N) from long-double cos/sin of an exactly reduced angle
-    std::vector<cplx<double>> tw(N);
+    // twiddles exp(-2 pi i k/N) from long-double cos/sin
+    std::vector<double> tw(2 * N);
+    std::vector<float> twf(2 * N);
     for (int k = 0; k < N; ++k) {
         const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)N;
-        tw[k] = cplx<double>{(double)cosl(ang), (double)sinl(ang)};
+        tw[2 * k] = (double)cosl(ang);
+        tw[2 * k + 1] = (double)sinl(ang);
+        twf[2 * k] = (float)cosl(ang);
+        twf[2 * k + 1] = (float)sinl(ang);
     }
-    if (hipMemcpy(h->tw, tw.data(), N * sizeof(cplx<double>), hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(h->tw, mixed ? (void*)twf.data() : (void*)tw.data(), N * 2 * rb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(h->dscal, 0, (8 + kRing) * sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(h->acc, 0, 3 * h->n * sizeof(double)) != hipSuccess) {
+        hipMemset(mixed ? (void*)h->accf : (void*)h->acc, 0, n3 * rb) != hipSuccess) {
         h->err = "initial copies failed";
         return bail(PM3D_EHIP);
     }
@@ -441,13 +587,18 @@ int pm3d_set_state(void* handle, const double* x, const double* p, int64_t n) {
     if (rc) return rc;
     if (x) {
         HCHK(h, hipMemcpyAsync(h->sbuf, x, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->st));
-        k_aos_to_soa<<<grid_for(n), kThreads, 0, h->st>>>(h->sbuf, h->x, n);
+        if (!h->mixed)
+            k_aos_to_soa<<<grid_for(n), kThreads, 0, h->st>>>(h->sbuf, h->x, n);
+        else
+            k_set_pos_mx<<<grid_for(n), kThreads, 0, h->st>>>(h->sbuf, h->cell, h->off, n, h->N, h->dx);
     }
     if (p) {
-        // the host value of p already contains every kick: drop a pending one
-        h->kick_pending = false;
+        h->kick_pending = false;  // the host value of p already contains every kick
         HCHK(h, hipMemcpyAsync(h->sbuf, p, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->st));
-        k_aos_to_soa<<<grid_for(n), kThreads, 0, h->st>>>(h->sbuf, h->p, n);
+        if (!h->mixed)
+            k_aos_to_soa<<<grid_for(n), kThreads, 0, h->st>>>(h->sbuf, h->p, n);
+        else
+            k_f64_to_f32_soa<<<grid_for(n), kThreads, 0, h->st>>>(h->sbuf, h->pf, n);
     }
     HCHK(h, hipStreamSynchronize(h->st));
     h->state_set = true;
@@ -461,13 +612,19 @@ int pm3d_get_state(void* handle, double* x, double* p) {
     int rc = ensure_sbuf(h, 3 * h->n * sizeof(double));
     if (rc) return rc;
     if (x) {
-        k_soa_to_aos<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->sbuf, h->n);
+        if (!h->mixed)
+            k_soa_to_aos<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->sbuf, h->n);
+        else
+            k_get_pos_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(h->cell, h->off, h->sbuf, h->n, h->dx, h->L);
         HCHK(h, hipMemcpyAsync(x, h->sbuf, 3 * h->n * sizeof(double), hipMemcpyDeviceToHost, h->st));
         HCHK(h, hipStreamSynchronize(h->st));
     }
     if (p) {
         if ((rc = flush_kick(h))) return rc;
-        k_soa_to_aos<<<grid_for(h->n), kThreads, 0, h->st>>>(h->p, h->sbuf, h->n);
+        if (!h->mixed)
+            k_soa_to_aos<<<grid_for(h->n), kThreads, 0, h->st>>>(h->p, h->sbuf, h->n);
+        else
+            k_f32_soa_to_f64<<<grid_for(h->n), kThreads, 0, h->st>>>(h->pf, h->sbuf, h->n);
         HCHK(h, hipMemcpyAsync(p, h->sbuf, 3 * h->n * sizeof(double), hipMemcpyDeviceToHost, h->st));
         HCHK(h, hipStreamSynchronize(h->st));
     }
@@ -498,16 +655,24 @@ int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho) {
     const size_t xb = 3 * (size_t)n * sizeof(double);
     int rc = ensure_sbuf(h, std::max(xb, (size_t)h->M * sizeof(double)));
     if (rc) return rc;
-    k_fill_u64<<<std::min<long long>(grid_for(2 * h->M), 4096), kThreads, 0, h->st>>>(h->mesh, 2 * h->M, 0ull);
+    const int W = h->wide ? 2 : 1;
+    k_fill_u64<<<grid_cap(W * h->M), kThreads, 0, h->st>>>(h->mesh, W * h->M, 0ull);
     if (n > 0) {
         HCHK(h, hipMemcpyAsync(h->sbuf, X, xb, hipMemcpyHostToDevice, h->st));
-        k_deposit_atomic<true><<<grid_for(n), kThreads, 0, h->st>>>(Strided{h->sbuf, 1, 3}, n, h->N, h->dx, h->mesh, h->qs,
-                                                                    h->qone);
+        if (h->wide)
+            k_deposit_atomic<true><<<grid_for(n), kThreads, 0, h->st>>>(Strided{h->sbuf, 1, 3}, n, h->N, h->dx, h->mesh,
+                                                                        h->qs, h->qone);
+        else
+            k_deposit_atomic<false><<<grid_for(n), kThreads, 0, h->st>>>(Strided{h->sbuf, 1, 3}, n, h->N, h->dx, h->mesh,
+                                                                         h->qs, h->qone);
     }
-    k_fixed_to_real<true><<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 0,
-                                                                                             1.0, h->sbuf);
+    if (h->wide)
+        k_fixed_to_real<true><<<grid_cap(h->M), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 0, 1.0, h->sbuf);
+    else
+        k_fixed_to_real<false><<<grid_cap(h->M), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 0, 1.0, h->sbuf);
     HCHK(h, hipMemcpyAsync(rho, h->sbuf, h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
+    h->serial += 1;  // the state's last deposit is no longer in the mesh
     return check_launch(h, "deposit");
 }
 
@@ -529,17 +694,6 @@ int pm3d_interp(void* handle, const double* F, const double* X, int64_t n, doubl
     return check_launch(h, "interp");
 }
 
-static int fetch_delta(Handle* h, double* delta, double* dmax) {
-    if (delta) {
-        HCHK(h, hipMemcpyAsync(delta, h->delta, h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
-    }
-    if (dmax) {
-        int rc = pm3d_delta_max_at(h, h->serial, dmax);
-        if (rc) return rc;
-    }
-    return 0;
-}
-
 int64_t pm3d_deposit_serial(void* handle) { return handle ? (int64_t)((Handle*)handle)->serial : -1; }
 
 int pm3d_delta_max_at(void* handle, int64_t serial, double* dmax) {
@@ -550,10 +704,9 @@ int pm3d_delta_max_at(void* handle, int64_t serial, double* dmax) {
     unsigned long long q = 0;
     HCHK(h, hipMemcpyAsync(&q, h->dscal + 8 + (serial % kRing), sizeof(q), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
-    const double nmean = (double)h->Np * h->Np * h->Np / ((double)h->N * h->N * h->N);
     double rmax;
     std::memcpy(&rmax, &q, sizeof(rmax));
-    *dmax = rmax / nmean - 1.0;
+    *dmax = rmax / h->nmean - 1.0;
     return 0;
 }
 
@@ -563,9 +716,7 @@ int pm3d_get_delta(void* handle, double* delta) {
     if (h->serial < 1) return fail(h, PM3D_ESTATE, "no deposit yet");
     int rc = ensure_sbuf(h, h->M * sizeof(double));
     if (rc) return rc;
-    const double nmean = (double)h->Np * h->Np * h->Np / ((double)h->N * h->N * h->N);
-    k_fixed_to_real<true><<<std::min<long long>(grid_for(h->M), 4096), kThreads, 0, h->st>>>(h->mesh, h->M, h->qscale, 1,
-                                                                                             nmean, h->sbuf);
+    mesh_to_delta(h, h->sbuf);
     HCHK(h, hipMemcpyAsync(delta, h->sbuf, h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
     return check_launch(h, "get_delta");
@@ -576,17 +727,25 @@ int pm3d_forces(void* handle, double a, double s, double sm2, double* acc, doubl
     if (!handle) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
     if (!h->state_set) return fail(h, PM3D_ESTATE, "state not set");
-    // the step's cached acceleration is left alone (pm3d.py:101-114 does not touch _acc):
-    // this call gathers into staging memory
-    int rc = ensure_sbuf(h, 6 * h->n * sizeof(double));
+    // the step's cached acceleration is left alone (pm3d.py:101-114 does not touch _acc): this
+    // call gathers into staging memory
+    const size_t rb = h->mixed ? 4 : 8;
+    int rc = ensure_sbuf(h, 3 * h->n * rb + std::max<size_t>(3 * h->n, h->M) * sizeof(double));
     if (rc) return rc;
-    if (delta && !h->delta && (rc = dalloc(h, &h->delta, h->M))) return rc;
-    if ((rc = forces_state(h, s, sm2, delta ? h->delta : nullptr, h->sbuf))) return rc;
+    double* stage = (double*)((char*)h->sbuf + 3 * h->n * rb);
+    if ((rc = forces_state(h, s, sm2, h->sbuf))) return rc;
     if (acc) {
-        k_soa_to_aos<<<grid_for(h->n), kThreads, 0, h->st>>>(h->sbuf, h->sbuf + 3 * h->n, h->n);
-        HCHK(h, hipMemcpyAsync(acc, h->sbuf + 3 * h->n, 3 * h->n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        if (!h->mixed)
+            k_soa_to_aos<<<grid_for(h->n), kThreads, 0, h->st>>>(h->sbuf, stage, h->n);
+        else
+            k_f32_soa_to_f64<<<grid_for(h->n), kThreads, 0, h->st>>>((const float*)h->sbuf, stage, h->n);
+        HCHK(h, hipMemcpyAsync(acc, stage, 3 * h->n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HCHK(h, hipStreamSynchronize(h->st));
     }
-    if ((rc = fetch_delta(h, delta, nullptr))) return rc;
+    if (delta) {
+        mesh_to_delta(h, stage);
+        HCHK(h, hipMemcpyAsync(delta, stage, h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    }
     HCHK(h, hipStreamSynchronize(h->st));
     int bad = 0;
     if ((rc = read_bad(h, &bad))) return rc;
@@ -598,33 +757,39 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
     Handle* h = (Handle*)handle;
     if (!h->state_set) return fail(h, PM3D_ESTATE, "state not set");
     int rc;
-    if (delta && !h->delta && (rc = dalloc(h, &h->delta, h->M))) return rc;
+    void* accbuf = h->mixed ? (void*)h->accf : (void*)h->acc;
     if (sc->need_forces || !h->acc_valid) {
         // the previous closing kick used the old acceleration: apply it before replacing acc
         if ((rc = flush_kick(h))) return rc;
-        if ((rc = forces_state(h, sc->s_old, sc->sm2, nullptr, h->acc))) return rc;
+        if ((rc = forces_state(h, sc->s_old, sc->sm2, accbuf))) return rc;
         h->acc_valid = true;
     }
     {
         // pending closing kick of the previous step (pm3d.py:127) + opening kick (:122) + drift
-        // (:123-124) in one pass, with the reference's two separate roundings of p
+        // (:123-124) in one pass; FP64 keeps the reference's two separate roundings of p
         Timed t(h, KC_KICK_DRIFT);
-        k_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->acc, h->n, h->kick_pending ? 1 : 0,
-                                                             h->pend_c1, h->pend_den, sc->c1, sc->den1, sc->da,
-                                                             sc->den_d, h->L, h->dscal + 1);
+        if (!h->mixed)
+            k_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->acc, h->n, h->kick_pending ? 1 : 0,
+                                                                 h->pend_c1, h->pend_den, sc->c1, sc->den1, sc->da,
+                                                                 sc->den_d, h->L, h->dscal + 1);
+        else
+            k_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(
+                h->cell, h->off, h->pf, h->accf, h->n, h->N, h->kick_pending ? 1 : 0,
+                (float)(h->kick_pending ? h->pend_c1 / h->pend_den : 0.0), (float)(sc->c1 / sc->den1),
+                (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1);
     }
     h->kick_pending = false;
     if ((rc = check_launch(h, "kick_drift"))) return rc;
     // the closing kick is deferred: the gather stores acc, and the kick is applied with the next
     // opening kick or before p is read
-    if ((rc = forces_state(h, sc->s_new, sc->sm2, delta ? h->delta : nullptr, h->acc))) return rc;
+    if ((rc = forces_state(h, sc->s_new, sc->sm2, accbuf))) return rc;
     h->acc_valid = true;
     h->kick_pending = true;
     h->pend_c1 = sc->c1;
     h->pend_den = sc->den2;
     if (delta || delta_max) {
-        if ((rc = fetch_delta(h, delta, delta_max))) return rc;
-        HCHK(h, hipStreamSynchronize(h->st));
+        if (delta && (rc = pm3d_get_delta(h, delta))) return rc;
+        if (delta_max && (rc = pm3d_delta_max_at(h, h->serial, delta_max))) return rc;
         int bad = 0;
         if ((rc = read_bad(h, &bad))) return rc;
         return bad;
@@ -635,74 +800,34 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
 int pm3d_rfftn(void* handle, const double* in, double* out_c) {
     if (!handle || !in || !out_c) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
-    const int N = h->N, NH = N / 2;
-    int rc = ensure_sbuf(h, h->M * sizeof(double));
-    if (rc) return rc;
-    HCHK(h, hipMemcpyAsync(h->sbuf, in, h->M * sizeof(double), hipMemcpyHostToDevice, h->st));
-    PM3D_DISPATCH(h, launch_fft_forward_tmpl, (const void*)h->sbuf, false, nullptr, 1.0);
-    PM3D_DISPATCH(h, launch_x_tmpl, 0, 0.0, 0.0);
-    if ((rc = check_launch(h, "rfftn"))) return rc;
-    const size_t specN = (size_t)N * N * NH;
-    std::vector<cplx<double>> hs(specN + (size_t)N * N);
-    HCHK(h, hipMemcpyAsync(hs.data(), h->spec, hs.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost, h->st));
-    HCHK(h, hipStreamSynchronize(h->st));
-    for (size_t r = 0; r < (size_t)N * N; ++r) {
-        std::memcpy(out_c + 2 * r * (NH + 1), &hs[r * NH], NH * sizeof(cplx<double>));
-        out_c[2 * (r * (NH + 1) + NH)] = hs[specN + r].re;
-        out_c[2 * (r * (NH + 1) + NH) + 1] = hs[specN + r].im;
-    }
-    return 0;
+    return h->mixed ? rfftn_impl<float>(h, in, out_c) : rfftn_impl<double>(h, in, out_c);
 }
 
 int pm3d_irfftn(void* handle, const double* in_c, double* out) {
     if (!handle || !in_c || !out) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
-    const int N = h->N, NH = N / 2;
-    const size_t specN = (size_t)N * N * NH;
-    std::vector<cplx<double>> hs(specN + (size_t)N * N);
-    for (size_t r = 0; r < (size_t)N * N; ++r) {
-        std::memcpy(&hs[r * NH], in_c + 2 * r * (NH + 1), NH * sizeof(cplx<double>));
-        hs[specN + r] = cplx<double>{in_c[2 * (r * (NH + 1) + NH)], in_c[2 * (r * (NH + 1) + NH) + 1]};
-    }
-    int rc = ensure_sbuf(h, h->M * sizeof(double));
-    if (rc) return rc;
-    HCHK(h, hipMemcpyAsync(h->spec, hs.data(), hs.size() * sizeof(cplx<double>), hipMemcpyHostToDevice, h->st));
-    PM3D_DISPATCH(h, launch_x_tmpl, 1, 0.0, 0.0);
-    PM3D_DISPATCH(h, launch_inverse_tmpl, h->spec, 1, h->sbuf);
-    if ((rc = check_launch(h, "irfftn"))) return rc;
-    HCHK(h, hipMemcpyAsync(out, h->sbuf, h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
-    HCHK(h, hipStreamSynchronize(h->st));
-    return 0;
+    return h->mixed ? irfftn_impl<float>(h, in_c, out) : irfftn_impl<double>(h, in_c, out);
 }
 
 int pm3d_poisson_gradient(void* handle, const double* delta, double s, double sm2, double* g) {
     if (!handle || !delta || !g) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
-    int rc = ensure_sbuf(h, h->M * sizeof(double));
-    if (rc) return rc;
-    HCHK(h, hipMemcpyAsync(h->sbuf, delta, h->M * sizeof(double), hipMemcpyHostToDevice, h->st));
-    PM3D_DISPATCH(h, launch_fft_forward_tmpl, (const void*)h->sbuf, false, nullptr, 1.0);
-    PM3D_DISPATCH(h, launch_x_tmpl, 2, s, sm2);
-    PM3D_DISPATCH(h, launch_inverse_tmpl, h->gspec, 3, h->grad);
-    if ((rc = check_launch(h, "poisson_gradient"))) return rc;
-    HCHK(h, hipMemcpyAsync(g, h->grad, 3 * h->M * sizeof(double), hipMemcpyDeviceToHost, h->st));
-    HCHK(h, hipStreamSynchronize(h->st));
-    return 0;
+    return h->mixed ? poisson_impl<float>(h, delta, s, sm2, g) : poisson_impl<double>(h, delta, s, sm2, g);
 }
 
 int pm3d_min_jacobian(void* handle, double* jmin) {
     if (!handle || !jmin) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
     if (!h->state_set) return fail(h, PM3D_ESTATE, "state not set");
-    k_fill_u64<<<1, kThreads, 0, h->st>>>(h->dscal + 2, 1, 0x7fffffffffffffffull);
-    k_min_jacobian<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, (long long*)(h->dscal + 2));
-    long long o = 0;
-    HCHK(h, hipMemcpyAsync(&o, h->dscal + 2, sizeof(o), hipMemcpyDeviceToHost, h->st));
+    const unsigned g = grid_for(h->n);
+    if (!h->mixed)
+        k_min_jacobian<<<g, kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, h->bmax);
+    else
+        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, (double)h->N / h->Np, h->bmax);
+    std::vector<double> hb(g);
+    HCHK(h, hipMemcpyAsync(hb.data(), h->bmax, g * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
-    const long long bits = o >= 0 ? o : (o ^ 0x7fffffffffffffffLL);
-    double v;
-    std::memcpy(&v, &bits, sizeof(v));
-    *jmin = v;
+    *jmin = *std::min_element(hb.begin(), hb.end());
     return check_launch(h, "min_jacobian");
 }
 

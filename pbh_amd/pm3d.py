@@ -173,8 +173,18 @@ class PM3D:
             self._kgrid = (kx, ky, kzz, k2)
         return self._kgrid
 
-    # -- initial data (pm3d.py:40-73): transforms and the lattice interpolation run on the device
+    # -- initial data (pm3d.py:40-73): transforms and the lattice interpolation run on the device,
+    # in FP64 (a mixed-precision PM3D builds its initial data with an FP64 helper handle)
     def make_ic(self, mu, profile_spline, e=0.0, p=0.0, tail_rms=0.0, tail_k=(2.0, 10.0)):
+        if self.precision != "fp64":
+            eng = PM3D(self.N, self.L, self.Np, smooth=self.smooth, precision="fp64")
+            eng.rng = self.rng
+            eng.make_ic(mu, profile_spline, e, p, tail_rms, tail_k)
+            self.set_state(eng.x, eng.p)
+            for k in ("q", "delta_lin", "r_ell_q", "A", "tail_rms_grid"):
+                setattr(self, k, getattr(eng, k))
+            self._acc_a = None
+            return self
         N, L = self.N, self.L
         kx, ky, kz, k2 = self._kgrids()
         xg = (np.arange(N) + 0.5) * self.dx - L / 2

