@@ -37,6 +37,17 @@ __device__ __forceinline__ double npy_rem(double a, double b) {
     return m;
 }
 
+// npy_rem for the drift, where v = x + dx_step stays within one box of [0, L): fmod is exact
+// (v itself, or v - L by Sterbenz), and only v < 0 takes the rounded `mod += L` of numpy.
+__device__ __forceinline__ double npy_rem_near(double v, double L) {
+    double m;
+    if (v >= 0.0 && v < L) m = v;
+    else if (v >= L && v < 2.0 * L) m = v - L;
+    else if (v < 0.0 && v > -L) m = v + L;
+    else return npy_rem(v, L);
+    return m == 0.0 ? 0.0 : m;  // numpy returns +0.0 for a zero remainder
+}
+
 // One axis of the cell-centred CIC stencil in FP64 (pm3d.py:76-79): g = X/dx - 0.5 with IEEE
 // division, i0 = floor(g), f = g - i0; returns (i0 mod N, (i0+1) mod N) with Python's
 // non-negative modulo.
@@ -49,6 +60,52 @@ __device__ __forceinline__ void cic_axis(double X, double dx, int N, int& lo, in
     if (m < 0) m += N;
     lo = (int)m;
     hi = (m + 1 == N) ? 0 : (int)(m + 1);
+}
+
+// Hot-path form of cic_axis with 32-bit integer arithmetic: identical results (fl is integral, so
+// g - fl == g - (double)(int64)fl), periodic wrap by conditional add/subtract, and the general
+// 64-bit modulo only for a position more than one box away from [0, L).
+__device__ __forceinline__ double div_dx(double X, double dx, double inv_dx) {
+    // X/dx correctly rounded: q0 = X*(1/dx) is within 1 ulp and one FMA correction step with the
+    // correctly rounded reciprocal gives the IEEE quotient (Markstein), outside under/overflow
+    // range; checked bit for bit against X/dx on 2.2e8 positions, cell centres and cell edges.
+    const double ax = fabs(X);
+    if (ax < 1e290 && (ax > 1e-290 || X == 0.0)) {
+        const double q0 = X * inv_dx;
+        return fma(fma(-q0, dx, X), inv_dx, q0);
+    }
+    return X / dx;
+}
+
+__device__ __forceinline__ void cic_axis_fast(double X, double dx, int N, int& lo, int& hi, double& f) {
+    const double g = div_dx(X, dx, 1.0 / dx) - 0.5;
+    const double fl = floor(g);
+    f = g - fl;
+    int m;
+    if (fabs(fl) < 1e9) {
+        m = (int)fl;
+        if (m < 0) m += N;
+        else if (m >= N) m -= N;
+        if ((unsigned)m >= (unsigned)N) {
+            m %= N;
+            if (m < 0) m += N;
+        }
+    } else {
+        long long t = (long long)fl % N;
+        m = (int)(t < 0 ? t + N : t);
+    }
+    lo = m;
+    hi = (m + 1 == N) ? 0 : m + 1;
+}
+
+// v in [-N, 2N) wrapped into [0, N) without an integer division.
+__device__ __forceinline__ int wrapN(int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); }
+
+// Exact conversion of an integral double in [0, 2^63) to uint64 with two 32-bit conversions.
+__device__ __forceinline__ unsigned long long u64_of_integral(double q) {
+    const unsigned int hi = (unsigned int)(q * 0x1p-32);
+    const unsigned int lo = (unsigned int)(q - (double)hi * 0x1p32);
+    return ((unsigned long long)hi << 32) | lo;
 }
 
 // Wave64 max / min reductions through DPP-free shuffles.

@@ -1,0 +1,195 @@
+// Owner-computes CIC deposit in column bricks (gfx950), shared by the FP64 and mixed modes.
+//
+// The mesh is cut into columns of BX x BY x N cells (the whole z axis).  The workgroup owning a
+// column reads every particle whose Lagrangian site lies in the column or in the one-site ring
+// around it in x and y ((BX+2) x (BY+2) rows of N sites: each row is N contiguous elements, so the
+// reads are fully coalesced), accumulates the CIC corners that land inside its column in LDS with
+// integer atomics, and writes the column once with plain stores (no global atomics, no clearing
+// pass).  This relies on Np == N and on each particle's base cell being its site or the site just
+// below it on every axis ("near").  A particle outside that range ("far") is recorded by the
+// workgroup that owns its site, in that workgroup's own segment of `far` (no global counter), and is
+// added afterwards by k_deposit_far_seg with global atomics.
+//
+// Integer accumulation commutes, so the result is bitwise identical for any schedule, and
+// identical to the reference-shaped atomic deposit.
+#pragma once
+#include "common.hpp"
+#include "particles.hpp"
+#include "particles_mixed.hpp"
+
+namespace pm3d {
+
+// Particle sources: raw loads of particle i (issued in batches, so that several sites' loads are in
+// flight per thread), then base cell, upper cell and the 8 fixed-point weights.
+struct SrcF64 {
+    const double* x;
+    long long n;
+    double dx, qs;
+    long long one;
+    int N;
+    struct Raw {
+        double v[3];
+    };
+    __device__ __forceinline__ Raw load(long long i) const { return Raw{{x[i], x[n + i], x[2 * n + i]}}; }
+    __device__ __forceinline__ void stencil(const Raw& r, int lo[3], int hi[3], unsigned long long q[8]) const {
+        double f[3], w[8];
+        long long qq[8];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cic_axis_fast(r.v[d], dx, N, lo[d], hi[d], f[d]);
+        cic_weights(f, w);
+        quantise8(f, w, qs, one, qq);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) q[c] = (unsigned long long)qq[c];
+    }
+};
+
+struct SrcMx {
+    const int* cell;
+    const float* off;
+    long long n;
+    int N;
+    struct Raw {
+        int c[3];
+        float f[3];
+    };
+    __device__ __forceinline__ Raw load(long long i) const {
+        return Raw{{cell[i], cell[n + i], cell[2 * n + i]}, {off[i], off[n + i], off[2 * n + i]}};
+    }
+    __device__ __forceinline__ void stencil(const Raw& r, int lo[3], int hi[3], unsigned long long q[8]) const {
+        float w[8];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = r.c[d];
+            hi[d] = lo[d] + 1 == N ? 0 : lo[d] + 1;
+        }
+        mx_weights(r.f, w);
+        mx_quantise8(r.f, w, q);
+    }
+};
+
+// Displacement of the base cell from the site along one axis (minimum image).
+__device__ __forceinline__ int site_offset(int lo, int site, int N) {
+    int dd = lo - site;
+    if (dd > N / 2) dd -= N;
+    if (dd < -N / 2) dd += N;
+    return dd;
+}
+
+template <class Src, int BX, int BY, int N, bool WIDE, int NT>
+__global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long* __restrict__ mesh,
+                                                    int* __restrict__ far, unsigned int* __restrict__ farcnt) {
+    constexpr int NCELL = BX * BY * N;
+    constexpr int EX = BX + 2, EY = BY + 2, NSITE = EX * EY * N;
+    __shared__ unsigned long long acc[NCELL];
+    __shared__ unsigned int carry[WIDE ? NCELL : 1];
+    __shared__ unsigned int nfar;
+    for (int t = threadIdx.x; t < NCELL; t += NT) {
+        acc[t] = 0ull;
+        if (WIDE) carry[t] = 0u;
+    }
+    if (threadIdx.x == 0) nfar = 0u;
+    __syncthreads();
+    constexpr int NBY = N / BY;
+    const int x0 = (blockIdx.x / NBY) * BX, y0 = (blockIdx.x % NBY) * BY;
+    int* myfar = far + (size_t)blockIdx.x * (BX * BY * N);
+    constexpr int U = 4;  // sites whose loads are issued together
+    for (int t0 = threadIdx.x; t0 < NSITE; t0 += NT * U) {
+        typename Src::Raw raw[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * NT;
+            if (t < NSITE) {
+                const int sz = t % N, ey = (t / N) % EY, ex = t / (N * EY);
+                const int sx = wrapN(x0 + ex - 1, N), sy = wrapN(y0 + ey - 1, N);
+                raw[u] = src.load(((long long)sx * N + sy) * N + sz);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * NT;
+        if (t >= NSITE) break;
+        const int sz = t % N, ey = (t / N) % EY, ex = t / (N * EY);
+        const int sx = wrapN(x0 + ex - 1, N), sy = wrapN(y0 + ey - 1, N);
+        const long long i = ((long long)sx * N + sy) * N + sz;
+        int lo[3], hi[3];
+        unsigned long long q[8];
+        src.stencil(raw[u], lo, hi, q);
+        const int ox = site_offset(lo[0], sx, N), oy = site_offset(lo[1], sy, N), oz = site_offset(lo[2], sz, N);
+        const bool is_far = ox < -1 || ox > 0 || oy < -1 || oy > 0 || oz < -1 || oz > 0;
+        const bool own = ex >= 1 && ex <= BX && ey >= 1 && ey <= BY;
+        const bool app = is_far && own;
+        const unsigned long long bal = __ballot(app);
+        if (bal) {  // one LDS atomic per wave, slots by prefix popcount
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)bal) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(&nfar, (unsigned int)__popcll(bal));
+            base = __shfl(base, leader, 64);
+            if (app) myfar[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)i;
+        }
+        if (is_far) continue;
+        // column-local x/y of the two stencil planes; a plane outside the column maps to >= BX/BY
+        const int rx[2] = {wrapN(lo[0] - x0, N), wrapN(hi[0] - x0, N)};
+        const int ry[2] = {wrapN(lo[1] - y0, N), wrapN(hi[1] - y0, N)};
+#pragma unroll
+        for (int cxy = 0; cxy < 4; ++cxy) {
+            const int cx = cxy >> 1, cy = cxy & 1;
+            if (rx[cx] < BX && ry[cy] < BY) {
+                const int col = (rx[cx] * BY + ry[cy]) * N;
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    const unsigned long long v = q[cxy * 2 + cz];
+                    if (v != 0ull) {
+                        const int cell = col + (cz ? hi[2] : lo[2]);
+                        const unsigned long long old = atomicAdd(&acc[cell], v);
+                        if (WIDE && old + v < old) atomicAdd(&carry[cell], 1u);
+                    }
+                }
+            }
+        }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) farcnt[blockIdx.x] = nfar;
+    for (int t = threadIdx.x; t < NCELL; t += NT) {
+        const int rz = t % N, ry = (t / N) % BY, rx = t / (N * BY);
+        const size_t cell = ((size_t)(x0 + rx) * N + (y0 + ry)) * N + rz;
+        if (WIDE)
+            *(ulonglong2*)(mesh + 2 * cell) = make_ulonglong2(acc[t], (unsigned long long)carry[t]);
+        else
+            mesh[cell] = acc[t];
+    }
+}
+
+// Far particles of every column: workgroup b adds segment b of `far` with global atomics.
+template <class Src, bool WIDE>
+__global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned long long* __restrict__ mesh,
+                                                              const int* __restrict__ far,
+                                                              const unsigned int* __restrict__ farcnt, int seg, int N) {
+    const unsigned int m = farcnt[blockIdx.x];
+    const int* mine = far + (size_t)blockIdx.x * seg;
+    for (unsigned int t = threadIdx.x; t < m; t += kThreads) {
+        int lo[3], hi[3];
+        unsigned long long q[8];
+        src.stencil(src.load(mine[t]), lo, hi, q);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (q[c] == 0ull) continue;
+            const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
+            fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, q[c]);
+        }
+    }
+}
+
+// Column shape per mesh size, cell width and LDS budget: the widest of 8x8, 4x8, 4x4, 2x4
+// columns whose LDS accumulator fits in BUDGET bytes (gfx950 has 160 KB of LDS per CU).
+template <int N, bool WIDE, int BUDGET = 131072> struct ColShape {
+    static constexpr int CELLB = WIDE ? 12 : 8;
+    static constexpr bool fits(int bx, int by) { return bx * by * N * CELLB <= BUDGET; }
+    static constexpr int BX = fits(8, 8) ? 8 : fits(4, 8) ? 4 : fits(4, 4) ? 4 : 2;
+    static constexpr int BY = fits(8, 8) ? 8 : fits(4, 8) ? 8 : fits(4, 4) ? 4 : 4;
+    static constexpr int NBLK = (N / BX) * (N / BY);
+    static_assert(BX * BY * N * CELLB <= 163840, "column does not fit in LDS");
+};
+
+}  // namespace pm3d

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kThreads) void k_cic_index(Strided X, long long n, 
     int lo[3], hi[3];
     double f[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cic_axis(X(d, i), dx, N, lo[d], hi[d], f[d]);
+    for (int d = 0; d < 3; ++d) cic_axis_fast(X(d, i), dx, N, lo[d], hi[d], f[d]);
     key[i] = ((long long)lo[0] * N + lo[1]) * N + lo[2];
 #pragma unroll
     for (int d = 0; d < 3; ++d) frac[i * 3 + d] = f[d];
@@ -48,7 +48,7 @@ __device__ __forceinline__ void quantise8(const double f[3], const double w[8], 
     long long sum = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        q[c] = (c == cmax) ? 0 : (long long)rint(w[c] * qs);
+        q[c] = (c == cmax) ? 0 : (long long)u64_of_integral(rint(w[c] * qs));
         sum += q[c];
     }
     q[cmax] = one - sum;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
     double f[3], w[8];
     long long q[8];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cic_axis(X(d, i), dx, N, lo[d], hi[d], f[d]);
+    for (int d = 0; d < 3; ++d) cic_axis_fast(X(d, i), dx, N, lo[d], hi[d], f[d]);
     cic_weights(f, w);
     quantise8(f, w, qs, one, q);
 #pragma unroll
@@ -106,13 +106,13 @@ __global__ __launch_bounds__(NT) void k_deposit_brick(const double* __restrict__
     const double* x2 = x + 2 * n;
     for (int t = threadIdx.x; t < EX * EY * EZ; t += NT) {
         const int ez = t % EZ, ey = (t / EZ) % EY, ex = t / (EZ * EY);
-        const int sx = (x0 + ex - HALO + N) % N, sy = (y0 + ey - HALO + N) % N, sz = (z0 + ez - HALO + N) % N;
+        const int sx = wrapN(x0 + ex - HALO, N), sy = wrapN(y0 + ey - HALO, N), sz = wrapN(z0 + ez - HALO, N);
         const long long i = ((long long)sx * N + sy) * N + sz;
         const double X[3] = {x[i], x1[i], x2[i]};
         int lo[3], hi[3];
         double f[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cic_axis(X[d], dx, N, lo[d], hi[d], f[d]);
+        for (int d = 0; d < 3; ++d) cic_axis_fast(X[d], dx, N, lo[d], hi[d], f[d]);
         // displacement of the base cell from the site, minimum image, per axis
         const int site[3] = {sx, sy, sz};
         bool is_far = false;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_deposit_brick(const double* __restrict__
             const int cx = ((c >> 2) ? hi[0] : lo[0]) - x0, cy = (((c >> 1) & 1) ? hi[1] : lo[1]) - y0,
                       cz = ((c & 1) ? hi[2] : lo[2]) - z0;
             // wrap relative coordinates into the brick frame (brick may touch the periodic seam)
-            const int rx = (cx + N) % N, ry = (cy + N) % N, rz = (cz + N) % N;
+            const int rx = wrapN(cx, N), ry = wrapN(cy, N), rz = wrapN(cz, N);
             if (rx < BX && ry < BY && rz < BZ && q[c] != 0) {
                 const int cell = (rx * BY + ry) * BZ + rz;
                 const unsigned long long old = atomicAdd(&acc[cell], (unsigned long long)q[c]);
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far(const double* __restri
         double f[3], w[8];
         long long q[8];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cic_axis(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
+        for (int d = 0; d < 3; ++d) cic_axis_fast(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
         cic_weights(f, w);
         quantise8(f, w, qs, one, q);
 #pragma unroll
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__
     int lo[3], hi[3];
     double f[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cic_axis(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
+    for (int d = 0; d < 3; ++d) cic_axis_fast(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
     const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kThreads) void k_gather1(Strided X, long long n, in
     int lo[3], hi[3];
     double f[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cic_axis(X(d, i), dx, N, lo[d], hi[d], f[d]);
+    for (int d = 0; d < 3; ++d) cic_axis_fast(X(d, i), dx, N, lo[d], hi[d], f[d]);
     const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
     double a = 0.0;
 #pragma unroll
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_kick_drift(double* __restrict__ x,
             double pd = p[k];
             if (pend) pd = pd + (c1p * ak) / denp;
             pd = pd + (c1 * ak) / den1;
-            const double xd = npy_rem(x[k] + (da * pd) / den_d, L);
+            const double xd = npy_rem_near(x[k] + (da * pd) / den_d, L);
             p[k] = pd;
             x[k] = xd;
             ok &= isfinite(pd) && isfinite(xd);
@@ -321,10 +321,10 @@ __global__ __launch_bounds__(kThreads) void k_min_jacobian(const double* __restr
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     double det = 1e300;
     if (i < n) {
-        const int a = (int)(i / ((long long)Np * Np)), b = (int)((i / Np) % Np), c = (int)(i % Np);
-        const long long nb[3] = {((long long)((a + 1) % Np) * Np + b) * Np + c,
-                                 ((long long)a * Np + (b + 1) % Np) * Np + c,
-                                 ((long long)a * Np + b) * Np + (c + 1) % Np};
+        const int ii = (int)i, a = ii / (Np * Np), b = (ii / Np) - a * Np, c = ii - (ii / Np) * Np;
+        const long long nb[3] = {((long long)(a + 1 == Np ? 0 : a + 1) * Np + b) * Np + c,
+                                 ((long long)a * Np + (b + 1 == Np ? 0 : b + 1)) * Np + c,
+                                 ((long long)a * Np + b) * Np + (c + 1 == Np ? 0 : c + 1)};
         double J[3][3];
 #pragma unroll
         for (int e = 0; e < 3; ++e) {

@@ -24,7 +24,7 @@ __device__ __forceinline__ void mx_quantise8(const float f[3], const float w[8],
     unsigned long long sum = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        q[c] = (c == cmax) ? 0ull : (unsigned long long)rintf(w[c] * 4294967296.0f);
+        q[c] = (c == cmax) ? 0ull : u64_of_integral((double)rintf(w[c] * 4294967296.0f));
         sum += q[c];
     }
     q[cmax] = (1ull << kNarrowFB) - sum;
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(NT) void k_deposit_brick_mx(MxPos P, int N, unsigne
     const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
     for (int t = threadIdx.x; t < EX * EY * EZ; t += NT) {
         const int ez = t % EZ, ey = (t / EZ) % EY, ex = t / (EZ * EY);
-        const int site[3] = {(x0 + ex - HALO + N) % N, (y0 + ey - HALO + N) % N, (z0 + ez - HALO + N) % N};
+        const int site[3] = {wrapN(x0 + ex - HALO, N), wrapN(y0 + ey - HALO, N), wrapN(z0 + ez - HALO, N)};
         const long long i = ((long long)site[0] * N + site[1]) * N + site[2];
         int lo[3];
         float f[3];
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(NT) void k_deposit_brick_mx(MxPos P, int N, unsigne
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int cx = (c >> 2) ? lo[0] + 1 : lo[0], cy = ((c >> 1) & 1) ? lo[1] + 1 : lo[1], cz = (c & 1) ? lo[2] + 1 : lo[2];
-            const int rx = (cx - x0 + N) % N, ry = (cy - y0 + N) % N, rz = (cz - z0 + N) % N;
+            const int rx = wrapN(cx - x0, N), ry = wrapN(cy - y0, N), rz = wrapN(cz - z0, N);
             if (rx < BX && ry < BY && rz < BZ && q[c] != 0) atomicAdd(&acc[(rx * BY + ry) * BZ + rz], q[c]);
         }
     }
@@ -256,10 +256,10 @@ __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, i
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     double det = 1e300;
     if (i < n) {
-        const int a = (int)(i / ((long long)Np * Np)), b = (int)((i / Np) % Np), c = (int)(i % Np);
-        const long long nb[3] = {((long long)((a + 1) % Np) * Np + b) * Np + c,
-                                 ((long long)a * Np + (b + 1) % Np) * Np + c,
-                                 ((long long)a * Np + b) * Np + (c + 1) % Np};
+        const int ii = (int)i, a = ii / (Np * Np), b = (ii / Np) - a * Np, c = ii - (ii / Np) * Np;
+        const long long nb[3] = {((long long)(a + 1 == Np ? 0 : a + 1) * Np + b) * Np + c,
+                                 ((long long)a * Np + (b + 1 == Np ? 0 : b + 1)) * Np + c,
+                                 ((long long)a * Np + b) * Np + (c + 1 == Np ? 0 : c + 1)};
         double J[3][3];
 #pragma unroll
         for (int e = 0; e < 3; ++e) {

@@ -24,6 +24,7 @@
 #include "fft.hpp"
 #include "particles.hpp"
 #include "particles_mixed.hpp"
+#include "deposit.hpp"
 
 using namespace pm3d;
 
@@ -72,7 +73,8 @@ struct Handle {
     // [8 + s % kRing] max rho (double bits) of deposit serial s
     unsigned long long* dscal = nullptr;
     long long serial = 0;
-    int* far = nullptr;
+    int* far = nullptr;                // far particles, one segment per deposit column
+    unsigned int* farcnt = nullptr;    // far count per column
     double* bmax = nullptr;  // per-block maxima / minima scratch
     size_t bmax_len = 0;
     double* sbuf = nullptr;  // staging for reference-shaped entry points
@@ -267,28 +269,47 @@ int ensure_sbuf(Handle* h, size_t bytes) {
     return 0;
 }
 
-// Deposit of the device state into h->mesh.  The brick path writes every cell; only the atomic
-// path (Np != N or N < 32) clears the mesh first.  nfar is cleared by the r2c pass of the previous
-// deposit (and at creation); this deposit's maximum is stored into its ring slot by the y pass.
+// Column deposit (deposit.hpp) for one mesh size.
+template <int N, bool WIDE, class Src>
+void deposit_columns(Handle* h, const Src& src) {
+    using S = ColShape<N, WIDE>;
+    {
+        Timed t(h, KC_DEPOSIT);
+        k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024><<<S::NBLK, 1024, 0, h->st>>>(src, h->mesh, h->far, h->farcnt);
+    }
+    Timed t(h, KC_DEPOSIT_FAR);
+    k_deposit_far_seg<Src, WIDE><<<S::NBLK, kThreads, 0, h->st>>>(src, h->mesh, h->far, h->farcnt, S::BX * S::BY * N, N);
+}
+
+template <bool WIDE, class Src>
+int deposit_columns_n(Handle* h, const Src& src) {
+    switch (h->N) {
+        case 16: deposit_columns<16, WIDE>(h, src); break;
+        case 32: deposit_columns<32, WIDE>(h, src); break;
+        case 64: deposit_columns<64, WIDE>(h, src); break;
+        case 128: deposit_columns<128, WIDE>(h, src); break;
+        case 256: deposit_columns<256, WIDE>(h, src); break;
+        case 512: deposit_columns<512, WIDE>(h, src); break;
+        default: return fail(h, PM3D_EUNSUPPORTED, "column deposit needs N in [16,512]");
+    }
+    return 0;
+}
+
+// Deposit of the device state into h->mesh.  The column path (Np == N, N >= 16) writes every cell;
+// the atomic path clears the mesh first.  This deposit's maximum is stored into its ring slot by the
+// y pass of the FFT.
 int deposit_state(Handle* h) {
-    const bool tiled = h->Np == h->N && h->N >= 32;  // 16^3 bricks with a 1-site halo must not wrap
+    const bool tiled = h->Np == h->N && h->N >= 16;  // a column plus its one-site ring must not wrap onto itself
     h->serial += 1;
     const int W = h->wide ? 2 : 1;
     if (!tiled) {
         Timed t(h, KC_FILL);
         k_fill_u64<<<grid_cap(W * h->M), kThreads, 0, h->st>>>(h->mesh, W * h->M, 0ull);
     }
-    unsigned int* nfar = (unsigned int*)(h->dscal + 0);
-    const unsigned gb = (unsigned)((h->N / 16) * (h->N / 16) * (h->N / 16));
+    int rc = 0;
     if (!h->mixed) {
         if (tiled) {
-            {
-                Timed t(h, KC_DEPOSIT);
-                k_deposit_brick<16, 16, 16, 1, true, 1024>
-                    <<<gb, 1024, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone, h->far, nfar);
-            }
-            Timed t(h, KC_DEPOSIT_FAR);
-            k_deposit_far<true><<<1024, kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, h->mesh, h->qs, h->qone, h->far, nfar);
+            rc = deposit_columns_n<true>(h, SrcF64{h->x, h->n, h->dx, h->qs, h->qone, h->N});
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic<true>
@@ -296,18 +317,13 @@ int deposit_state(Handle* h) {
         }
     } else {
         if (tiled) {
-            {
-                Timed t(h, KC_DEPOSIT);
-                k_deposit_brick_mx<16, 16, 16, 1, 1024><<<gb, 1024, 0, h->st>>>(mxpos(h), h->N, h->mesh, h->far, nfar);
-            }
-            Timed t(h, KC_DEPOSIT_FAR);
-            k_deposit_far_mx<<<1024, kThreads, 0, h->st>>>(mxpos(h), h->N, h->mesh, h->far, nfar);
+            rc = deposit_columns_n<false>(h, SrcMx{h->cell, h->off, h->n, h->N});
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, h->mesh);
         }
     }
-    return check_launch(h, "deposit");
+    return rc ? rc : check_launch(h, "deposit");
 }
 
 // forces at the current positions: deposit, Poisson, gather into acc_out (SoA 3n, state precision).
@@ -350,7 +366,7 @@ int read_bad(Handle* h, int* bad) {
 
 void free_all(Handle* h) {
     void* bufs[] = {h->x, h->p, h->acc, h->cell, h->off, h->pf, h->accf, h->mesh, h->spec,
-                    h->gspec, h->grad, h->tw, h->dscal, h->far, h->bmax, h->sbuf};
+                    h->gspec, h->grad, h->tw, h->dscal, h->far, h->farcnt, h->bmax, h->sbuf};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& u : h->ev_used) {
@@ -531,7 +547,8 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
     if ((rc = dalloc(h, &h->mesh, (h->wide ? 2 : 1) * h->M)) || (rc = dalloc_bytes(h, &h->spec, specN * 2 * rb)) ||
         (rc = dalloc_bytes(h, &h->gspec, 3 * specN * 2 * rb)) || (rc = dalloc_bytes(h, &h->grad, 3 * h->M * rb)) ||
         (rc = dalloc_bytes(h, &h->tw, (size_t)N * 2 * rb)) || (rc = dalloc(h, &h->dscal, 8 + kRing)) ||
-        (rc = dalloc(h, &h->bmax, h->bmax_len)) || (rc = dalloc(h, &h->far, (size_t)h->n)))
+        (rc = dalloc(h, &h->bmax, h->bmax_len)) || (rc = dalloc(h, &h->far, (size_t)h->n)) ||
+        (rc = dalloc(h, &h->farcnt, (size_t)N * N)))
         return bail(rc);
     // twiddles exp(-2 pi i k/N) from long-double cos/sin
     std::vector<double> tw(2 * N);
