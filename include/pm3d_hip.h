@@ -36,7 +36,7 @@ extern "C" {
 #define PM3D_ESTATE (-5)
 
 typedef struct pm3d_cfg {
-    int32_t N;          /* mesh cells per side: power of two in [8, 512] */
+    int32_t N;          /* mesh cells per side: power of two in [8, 512], or 48 or 96 */
     int32_t Np;         /* particles per side (n = Np^3) */
     double L;           /* periodic box side */
     double smooth;      /* Gaussian force smoothing length in cells, 0 = none (pm3d.py:27,107-108) */
@@ -118,6 +118,39 @@ int pm3d_min_jacobian(void* handle, double* jmin);
 int pm3d_set_timing(void* handle, int enable);
 int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax);
 const char* pm3d_kernel_names(void);  /* comma-separated */
+
+/* ---------------------------------------------------------------------------------------------
+ * Device-resident schedules and batched graph replay.  No counterpart in the reference (its driver
+ * loops over PM3D.step from Python, pbhgr/scripts/pm3d_peak.py:26-33); these entry points run the
+ * same step for many boxes at once.
+ *
+ * A schedule is a table of per-step host scalars (the same expressions as pm3d_step_scalars,
+ * evaluated in Python); entry k also carries the previous step's closing kick (c1p, denp).  The
+ * device keeps a step counter; every scheduled step reads entry [counter], records the minimum of
+ * det(dx/dq) of its new positions in a history and advances the counter.  A schedule continues a
+ * stepped state (the opening acceleration and a pending closing kick must exist: run one
+ * pm3d_step first).  Steps past the end of the table leave the state unchanged. */
+typedef struct pm3d_sched_entry {
+    double c1p, denp;  /* closing kick of the previous step: p += (c1p*acc)/denp */
+    double c1, den1;   /* opening kick */
+    double da, den_d;  /* drift */
+    double den2;       /* this step's closing kick denominator (kick applied by the next entry) */
+    double s, sm2;     /* -1.5*H_I**2/a_new, (smooth*dx)**2 */
+} pm3d_sched_entry;
+
+int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps);
+int pm3d_schedule_step(void* handle);                             /* one scheduled step on the handle's stream */
+int pm3d_schedule_status(void* handle, int* done, double* jhist); /* steps done, jmin history (done entries) */
+int pm3d_schedule_end(void* handle, int* done);                   /* hand the state back to pm3d_step */
+
+/* A batch replays one captured hipGraph holding one scheduled step of every member as
+ * independent branches (members of any mesh size and precision). */
+int pm3d_batch_create(void** batch);
+int pm3d_batch_destroy(void* batch);
+const char* pm3d_batch_last_error(void* batch);
+int pm3d_batch_set(void* batch, void* const* handles, int count); /* (re)capture for these members */
+int pm3d_batch_run(void* batch, int nsteps);                      /* replay nsteps times, asynchronous */
+int pm3d_batch_synchronize(void* batch);
 
 #ifdef __cplusplus
 }

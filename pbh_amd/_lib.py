@@ -32,6 +32,11 @@ class Pm3dStepScalars(ctypes.Structure):
                 ("sm2", c_double), ("need_forces", c_int32), ("pad", c_int32)]
 
 
+class Pm3dSchedEntry(ctypes.Structure):
+    _fields_ = [("c1p", c_double), ("denp", c_double), ("c1", c_double), ("den1", c_double),
+                ("da", c_double), ("den_d", c_double), ("den2", c_double), ("s", c_double), ("sm2", c_double)]
+
+
 PD = POINTER(c_double)
 PI64 = POINTER(c_int64)
 
@@ -59,6 +64,16 @@ SIGNATURES = {
     "pm3d_set_timing": (c_int, [c_void_p, c_int]),
     "pm3d_get_timings": (c_int, [c_void_p, PI64, PD, c_int]),
     "pm3d_kernel_names": (c_char_p, []),
+    "pm3d_set_schedule": (c_int, [c_void_p, POINTER(Pm3dSchedEntry), c_int]),
+    "pm3d_schedule_step": (c_int, [c_void_p]),
+    "pm3d_schedule_status": (c_int, [c_void_p, POINTER(c_int), PD]),
+    "pm3d_schedule_end": (c_int, [c_void_p, POINTER(c_int)]),
+    "pm3d_batch_create": (c_int, [POINTER(c_void_p)]),
+    "pm3d_batch_destroy": (c_int, [c_void_p]),
+    "pm3d_batch_last_error": (c_char_p, [c_void_p]),
+    "pm3d_batch_set": (c_int, [c_void_p, POINTER(c_void_p), c_int]),
+    "pm3d_batch_run": (c_int, [c_void_p, c_int]),
+    "pm3d_batch_synchronize": (c_int, [c_void_p]),
 }
 
 _lib = None

@@ -7,6 +7,23 @@ namespace pm3d {
 
 constexpr int kThreads = 256;  // 4 wave64 per workgroup in every kernel
 
+// Device-resident step schedule: entry `counter` of the table holds the scalars of the next step,
+// so one captured graph of a step replays for every step.  The FP32 factors are folded once on the
+// host for the mixed mode.
+struct DevStep {
+    double c1p, denp;   // pending closing kick of the previous step: p += (c1p*acc)/denp
+    double c1, den1;    // opening kick: p += (c1*acc)/den1
+    double da, den_d;   // drift: x += (da*p)/den_d
+    double den2;        // closing kick of this step (applied with the next opening kick)
+    double s, sm2;      // Green's function factor -1.5*H_I^2/a_new and (smooth*dx)^2
+    float kp, k1, sd, pad;  // mixed mode: c1p/denp, c1/den1, da/(den_d*dx)
+};
+struct DevSched {
+    int counter;  // next entry; advanced by the last kernel of each step
+    int nsteps;   // entries in the table
+    int pad0, pad1;
+};
+
 template <typename R>
 struct alignas(2 * sizeof(R)) cplx {
     R re, im;

@@ -1,7 +1,7 @@
 // Hand-written Stockham FFT for the periodic Poisson solve (gfx950).
 //
 // The 3D real-to-complex transform of an N^3 mesh is done in three pencil passes, each one a
-// kernel that stages P pencils in LDS, runs a radix-4 (+ one radix-2) Stockham FFT there and
+// kernel that stages P pencils in LDS, runs a mixed radix-4/2/3 Stockham FFT there and
 // writes back coalesced rows:
 //   z pass  (contiguous rows): real N -> N/2 complex FFT + even/odd split  -> half spectrum
 //   y pass  (stride N/2)     : complex N, in place
@@ -20,72 +20,98 @@
 
 namespace pm3d {
 
-// Pencils per workgroup for each pass: every LDS ping-pong buffer holds about 16 KB.
+// Pencils per workgroup for each pass: every LDS ping-pong buffer holds about 16 KB, and the count
+// is a power of two that divides the number of pencils (N is 2^k or 3*2^k).
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
+constexpr int pow2_dividing(int cap, int total) {  // largest power of two <= cap dividing total
+    return (cap <= 1) ? 1 : (total % cap == 0 ? cap : pow2_dividing(cap / 2, total));
+}
 template <typename R, int N> struct Plan {
     static constexpr int EB = 16384 / (2 * (int)sizeof(R));  // complex elements per buffer
-    static constexpr int PZ = cmin(2 * EB / N, N * N);       // r2c rows (N/2-point FFTs)
-    static constexpr int PY = cmin(EB / N, N / 2);           // y columns
-    static constexpr int PX = cmin(EB / N, N / 2);           // x columns (fused Poisson pass)
-    static constexpr int PC = cmin(EB / N, N * N);           // c2r rows (3 components each)
+    static constexpr int PZ = pow2_dividing(pow2_floor(cmin(2 * EB / N, N * N)), N * N);  // r2c rows
+    static constexpr int PY = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // y columns
+    static constexpr int PX = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // x columns
+    static constexpr int PC = pow2_dividing(pow2_floor(cmin(EB / N, N * N)), N * N);      // c2r rows
 };
 
-// In-LDS Stockham FFT of P pencils of NC points (unnormalised; forward = exp(-2 pi i jk/NC)).
-// Input in `a`, result in the returned buffer.  Caller synchronises before the call; the call
-// ends with a barrier.  tw[k] = exp(-2 pi i k / (NC*TWS)).
+// Stockham pass of radix RADIX on P pencils (see lds_fft).
+template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
+__device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, int Ns, const cplx<R>* __restrict__ tw) {
+    constexpr int PP = P + 1, M = NC / RADIX;
+    const int s = TWS * (NC / (RADIX * Ns));
+    for (int t = threadIdx.x; t < P * M; t += kThreads) {
+        const int c = t % P, j = t / P;
+        const int k = j % Ns;
+        cplx<R> v[RADIX];
+#pragma unroll
+        for (int r = 0; r < RADIX; ++r) v[r] = src[(j + r * M) * PP + c];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < RADIX; ++r) {
+                cplx<R> w = tw[r * k * s];
+                if (INV) w = cconj(w);
+                v[r] = cmul(v[r], w);
+            }
+        }
+        cplx<R> y[RADIX];
+        if constexpr (RADIX == 4) {
+            const cplx<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]), a2 = cadd(v[1], v[3]);
+            cplx<R> a3 = csub(v[1], v[3]);
+            a3 = INV ? cplx<R>{-a3.im, a3.re} : cplx<R>{a3.im, -a3.re};
+            y[0] = cadd(a0, a2);
+            y[1] = cadd(a1, a3);
+            y[2] = csub(a0, a2);
+            y[3] = csub(a1, a3);
+        } else if constexpr (RADIX == 2) {
+            y[0] = cadd(v[0], v[1]);
+            y[1] = csub(v[0], v[1]);
+        } else {  // radix 3: X1,2 = v0 - (v1+v2)/2 -/+ i sin(2pi/3) (v1-v2) (forward)
+            const R h = (R)0.86602540378443864676;  // sin(2 pi / 3)
+            const cplx<R> t1 = cadd(v[1], v[2]);
+            const cplx<R> t2 = cplx<R>{v[0].re - (R)0.5 * t1.re, v[0].im - (R)0.5 * t1.im};
+            const cplx<R> d = csub(v[1], v[2]);
+            const cplx<R> t3 = INV ? cplx<R>{-h * d.im, h * d.re} : cplx<R>{h * d.im, -h * d.re};  // -/+ i h d
+            y[0] = cadd(v[0], t1);
+            y[1] = cadd(t2, t3);
+            y[2] = csub(t2, t3);
+        }
+        const int dd = (j / Ns) * Ns * RADIX + k;
+#pragma unroll
+        for (int r = 0; r < RADIX; ++r) dst[(dd + r * Ns) * PP + c] = y[r];
+    }
+    __syncthreads();
+}
+
+constexpr int count_factor(int n, int f) { return (n % f == 0) ? 1 + count_factor(n / f, f) : 0; }
+
+// In-LDS Stockham FFT of P pencils of NC points (NC = 2^a 3^b, unnormalised; forward =
+// exp(-2 pi i jk/NC)).  Radix-4 passes, then one radix-2 pass if a is odd, then radix-3 passes.
+// Input in `a`, result in the returned buffer.  Caller synchronises before the call; the call ends
+// with a barrier.  tw[k] = exp(-2 pi i k / (NC*TWS)).
 template <typename R, int NC, int P, bool INV, int TWS>
 __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R>* __restrict__ tw) {
-    constexpr int PP = P + 1;
+    constexpr int P2 = count_factor(NC, 2), P3 = count_factor(NC, 3);
+    static_assert((1 << P2) * (P3 == 0 ? 1 : (P3 == 1 ? 3 : (P3 == 2 ? 9 : 27))) == NC, "NC must be 2^a 3^b, b <= 3");
     cplx<R>* src = a;
     cplx<R>* dst = b;
     int Ns = 1;
 #pragma unroll
-    for (; Ns * 4 <= NC; Ns *= 4) {
-        const int s = TWS * (NC / (4 * Ns));
-        for (int t = threadIdx.x; t < P * (NC / 4); t += kThreads) {
-            const int c = t % P, j = t / P;
-            const int k = j & (Ns - 1);
-            cplx<R> v0 = src[j * PP + c];
-            cplx<R> v1 = src[(j + NC / 4) * PP + c];
-            cplx<R> v2 = src[(j + NC / 2) * PP + c];
-            cplx<R> v3 = src[(j + 3 * NC / 4) * PP + c];
-            if (Ns > 1) {
-                cplx<R> w1 = tw[k * s], w2 = tw[2 * k * s], w3 = tw[3 * k * s];
-                if (INV) { w1 = cconj(w1); w2 = cconj(w2); w3 = cconj(w3); }
-                v1 = cmul(v1, w1);
-                v2 = cmul(v2, w2);
-                v3 = cmul(v3, w3);
-            }
-            const cplx<R> a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
-            cplx<R> a3 = csub(v1, v3);
-            a3 = INV ? cplx<R>{-a3.im, a3.re} : cplx<R>{a3.im, -a3.re};
-            const int d = (j / Ns) * Ns * 4 + k;
-            dst[d * PP + c] = cadd(a0, a2);
-            dst[(d + Ns) * PP + c] = cadd(a1, a3);
-            dst[(d + 2 * Ns) * PP + c] = csub(a0, a2);
-            dst[(d + 3 * Ns) * PP + c] = csub(a1, a3);
-        }
-        __syncthreads();
+    for (int q = 0; q < P2 / 2; ++q) {
+        stockham_pass<R, NC, P, INV, TWS, 4>(src, dst, Ns, tw);
+        Ns *= 4;
         cplx<R>* tmp = src; src = dst; dst = tmp;
     }
-    if (Ns < NC) {  // one radix-2 pass when log2(NC) is odd
-        const int s = TWS * (NC / (2 * Ns));
-        for (int t = threadIdx.x; t < P * (NC / 2); t += kThreads) {
-            const int c = t % P, j = t / P;
-            const int k = j & (Ns - 1);
-            cplx<R> v0 = src[j * PP + c];
-            cplx<R> v1 = src[(j + NC / 2) * PP + c];
-            if (Ns > 1) {
-                cplx<R> w = tw[k * s];
-                if (INV) w = cconj(w);
-                v1 = cmul(v1, w);
-            }
-            const int d = (j / Ns) * Ns * 2 + k;
-            dst[d * PP + c] = cadd(v0, v1);
-            dst[(d + Ns) * PP + c] = csub(v0, v1);
-        }
-        __syncthreads();
-        src = dst;
+    if (P2 & 1) {
+        stockham_pass<R, NC, P, INV, TWS, 2>(src, dst, Ns, tw);
+        Ns *= 2;
+        cplx<R>* tmp = src; src = dst; dst = tmp;
+    }
+#pragma unroll
+    for (int q = 0; q < P3; ++q) {
+        stockham_pass<R, NC, P, INV, TWS, 3>(src, dst, Ns, tw);
+        Ns *= 3;
+        cplx<R>* tmp = src; src = dst; dst = tmp;
     }
     return src;
 }
@@ -133,7 +159,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
     for (int t = threadIdx.x; t < P * NC; t += kThreads) {
         const int c = t / NC, k = t - c * NC;
         const cplx<R> zk = Z[k * PP + c];
-        const cplx<R> zm = Z[((NC - k) & (NC - 1)) * PP + c];
+        const cplx<R> zm = Z[(k == 0 ? 0 : NC - k) * PP + c];
         // even part E = (Z_k + conj Z_{-k})/2, odd part O = (Z_k - conj Z_{-k})/(2i)
         const R er = (zk.re + zm.re) * (R)0.5, ei = (zk.im - zm.im) * (R)0.5;
         const R orr = (zk.im + zm.im) * (R)0.5, oi = (zm.re - zk.re) * (R)0.5;
@@ -208,7 +234,14 @@ template <typename R, int N, int MODE>
 __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
                                                      cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
                                                      size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
-                                                     double kval, double s, double sm2, int smooth_on) {
+                                                     double kval, double s, double sm2, int smooth_on,
+                                                     const DevSched* __restrict__ sch, const DevStep* __restrict__ tab) {
+    if (MODE == 2 && sch) {  // Green's function factor of the scheduled step (last entry past the end)
+        const int k = sch->counter < sch->nsteps ? sch->counter : sch->nsteps - 1;
+        s = tab[k].s;
+        sm2 = tab[k].sm2;
+        smooth_on = sm2 > 0.0 ? 1 : 0;
+    }
     constexpr int P = Plan<R, N>::PX, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
     constexpr int EPT = (N * P + kThreads - 1) / kThreads;
     __shared__ cplx<R> A[N * PP], B[N * PP];

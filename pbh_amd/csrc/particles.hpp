@@ -248,10 +248,20 @@ __global__ __launch_bounds__(kThreads) void k_gather1(Strided X, long long n, in
 // x = (x + (da*p)/den_d) % L (pm3d.py:123-124).  Non-finite results are counted in *bad.
 // With `pend`, the previous step's deferred closing kick p += (c1p*acc)/denp (pm3d.py:127) is
 // applied first, as a separate rounding.
+// With a schedule (sch != nullptr) the scalars come from tab[sch->counter] and the pending kick is
+// always applied; past the end of the table the step does nothing.
 __global__ __launch_bounds__(kThreads) void k_kick_drift(double* __restrict__ x, double* __restrict__ p,
                                                          const double* __restrict__ acc, long long n, int pend,
                                                          double c1p, double denp, double c1, double den1, double da,
-                                                         double den_d, double L, unsigned long long* __restrict__ bad) {
+                                                         double den_d, double L, unsigned long long* __restrict__ bad,
+                                                         const DevSched* __restrict__ sch, const DevStep* __restrict__ tab) {
+    if (sch) {
+        const int k = sch->counter;
+        if (k >= sch->nsteps) return;
+        const DevStep e = tab[k];
+        pend = 1;
+        c1p = e.c1p; denp = e.denp; c1 = e.c1; den1 = e.den1; da = e.da; den_d = e.den_d;
+    }
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     bool ok = true;
     if (i < n) {

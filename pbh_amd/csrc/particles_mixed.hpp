@@ -167,7 +167,17 @@ __global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const f
 __global__ __launch_bounds__(kThreads) void k_kick_drift_mx(int* __restrict__ cell, float* __restrict__ off,
                                                             float* __restrict__ p, const float* __restrict__ acc,
                                                             long long n, int N, int pend, float kp, float k1, float sd,
-                                                            unsigned long long* __restrict__ bad) {
+                                                            unsigned long long* __restrict__ bad,
+                                                            const DevSched* __restrict__ sch,
+                                                            const DevStep* __restrict__ tab) {
+    if (sch) {  // scalars from the device schedule (see k_kick_drift)
+        const int k = sch->counter;
+        if (k >= sch->nsteps) return;
+        pend = 1;
+        kp = tab[k].kp;
+        k1 = tab[k].k1;
+        sd = tab[k].sd;
+    }
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     bool ok = true;
     if (i < n) {
@@ -247,6 +257,22 @@ __global__ __launch_bounds__(kThreads) void k_f32_soa_to_f64(const float* __rest
     if (i >= n) return;
 #pragma unroll
     for (int d = 0; d < 3; ++d) a[i * 3 + d] = (double)s[d * n + i];
+}
+
+// Last kernel of a scheduled step: reduce the Jacobian kernel's per-block minima into
+// jhist[counter] and advance the schedule.
+__global__ __launch_bounds__(kThreads) void k_sched_advance(DevSched* __restrict__ sch, const double* __restrict__ bmin,
+                                                            int nb, double* __restrict__ jhist) {
+    double m = 1e300;
+    for (int t = threadIdx.x; t < nb; t += kThreads) m = fmin(m, bmin[t]);
+    m = block_min(m);
+    if (threadIdx.x == 0) {
+        const int k = sch->counter;
+        if (k < sch->nsteps) {
+            jhist[k] = m;
+            sch->counter = k + 1;
+        }
+    }
 }
 
 // min det(dx/dq) over the Lagrangian lattice from cell + offset (differences exact in cells).

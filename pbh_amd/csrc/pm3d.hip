@@ -79,6 +79,13 @@ struct Handle {
     size_t bmax_len = 0;
     double* sbuf = nullptr;  // staging for reference-shaped entry points
     size_t sbuf_bytes = 0;
+    // device step schedule for graph replay
+    DevSched* sched = nullptr;
+    DevStep* tab = nullptr;
+    double* jhist = nullptr;
+    int tab_cap = 0;
+    std::vector<DevStep> htab;
+    bool use_sched = false;  // set while a batch graph is being captured
     bool state_set = false;
     bool acc_valid = false;
     bool kick_pending = false;
@@ -187,13 +194,16 @@ struct Fft {
         const unsigned g = N * (N / 2 / P) + N / P;
         const double kval = 1.0 / ((double)N * h->dx);
         Timed t(h, KC_X_POISSON);
+        const DevSched* sch = h->use_sched ? h->sched : nullptr;
         if (mode == 0)
-            k_x_pass<R, N, 0><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0);
+            k_x_pass<R, N, 0><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0,
+                                                         nullptr, nullptr);
         else if (mode == 1)
-            k_x_pass<R, N, 1><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0);
+            k_x_pass<R, N, 1><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0,
+                                                         nullptr, nullptr);
         else
             k_x_pass<R, N, 2><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s, sm2,
-                                                         sm2 > 0.0 ? 1 : 0);
+                                                         sm2 > 0.0 ? 1 : 0, sch, h->tab);
     }
 
     // inverse y (ncomp spectra from base) + c2r z into out (ncomp interleaved)
@@ -234,12 +244,14 @@ int fft_call(Handle* h, int op, const void* in, bool fixed, int mode, double s, 
         PM3D_FFT_CASE(8)
         PM3D_FFT_CASE(16)
         PM3D_FFT_CASE(32)
+        PM3D_FFT_CASE(48)
         PM3D_FFT_CASE(64)
+        PM3D_FFT_CASE(96)
         PM3D_FFT_CASE(128)
         PM3D_FFT_CASE(256)
         PM3D_FFT_CASE(512)
         default:
-            return fail(h, PM3D_EUNSUPPORTED, "N must be a power of two in [8,512]");
+            return fail(h, PM3D_EUNSUPPORTED, "unsupported mesh size");
     }
 #undef PM3D_FFT_CASE
     return 0;
@@ -286,7 +298,9 @@ int deposit_columns_n(Handle* h, const Src& src) {
     switch (h->N) {
         case 16: deposit_columns<16, WIDE>(h, src); break;
         case 32: deposit_columns<32, WIDE>(h, src); break;
+        case 48: deposit_columns<48, WIDE>(h, src); break;
         case 64: deposit_columns<64, WIDE>(h, src); break;
+        case 96: deposit_columns<96, WIDE>(h, src); break;
         case 128: deposit_columns<128, WIDE>(h, src); break;
         case 256: deposit_columns<256, WIDE>(h, src); break;
         case 512: deposit_columns<512, WIDE>(h, src); break;
@@ -365,8 +379,8 @@ int read_bad(Handle* h, int* bad) {
 }
 
 void free_all(Handle* h) {
-    void* bufs[] = {h->x, h->p, h->acc, h->cell, h->off, h->pf, h->accf, h->mesh, h->spec,
-                    h->gspec, h->grad, h->tw, h->dscal, h->far, h->farcnt, h->bmax, h->sbuf};
+    void* bufs[] = {h->x, h->p, h->acc, h->cell, h->off, h->pf, h->accf, h->mesh, h->spec, h->gspec,
+                    h->grad, h->tw, h->dscal, h->far, h->farcnt, h->bmax, h->sbuf, h->sched, h->tab, h->jhist};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& u : h->ev_used) {
@@ -473,6 +487,46 @@ int poisson_impl(Handle* h, const double* delta, double s, double sm2, double* g
     return 0;
 }
 
+// One step driven by the device schedule (for graph capture): kick+drift with the scheduled
+// scalars, forces, Jacobian minimum, schedule advance.  Launches only; no host synchronisation.
+int launch_sched_step(Handle* h) {
+    void* accbuf = h->mixed ? (void*)h->accf : (void*)h->acc;
+    if (!h->mixed)
+        k_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->acc, h->n, 1, 0, 1, 0, 1, 0, 1, h->L,
+                                                             h->dscal + 1, h->sched, h->tab);
+    else
+        k_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(h->cell, h->off, h->pf, h->accf, h->n, h->N, 1, 0, 0, 0,
+                                                                h->dscal + 1, h->sched, h->tab);
+    h->use_sched = true;
+    int rc = forces_state(h, 0.0, 0.0, accbuf);
+    h->use_sched = false;
+    if (rc) return rc;
+    const unsigned g = grid_for(h->n);
+    if (!h->mixed)
+        k_min_jacobian<<<g, kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, h->bmax);
+    else
+        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, (double)h->N / h->Np, h->bmax);
+    k_sched_advance<<<1, kThreads, 0, h->st>>>(h->sched, h->bmax, (int)g, h->jhist);
+    return check_launch(h, "scheduled step");
+}
+
+struct Batch {
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+    std::vector<Handle*> members;
+    std::string err;
+};
+
+void batch_clear_graph(Batch* b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
 }  // namespace
 
 // ================================================================================ C-ABI
@@ -487,8 +541,8 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
     const int N = cfg->N;
     int lg = 0;
     while ((1 << lg) < N) ++lg;
-    if (N < 8 || N > 512 || (1 << lg) != N) {
-        g_create_error = "N must be a power of two in [8,512]";
+    if (!((N >= 8 && N <= 512 && (1 << lg) == N) || N == 48 || N == 96)) {
+        g_create_error = "N must be a power of two in [8,512], or 48 or 96";
         return PM3D_EUNSUPPORTED;
     }
     if (cfg->Np < 1 || cfg->Np > 1024 || !(cfg->L > 0.0) || !(cfg->smooth >= 0.0)) {
@@ -788,12 +842,12 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
         if (!h->mixed)
             k_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->acc, h->n, h->kick_pending ? 1 : 0,
                                                                  h->pend_c1, h->pend_den, sc->c1, sc->den1, sc->da,
-                                                                 sc->den_d, h->L, h->dscal + 1);
+                                                                 sc->den_d, h->L, h->dscal + 1, nullptr, nullptr);
         else
             k_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(
                 h->cell, h->off, h->pf, h->accf, h->n, h->N, h->kick_pending ? 1 : 0,
                 (float)(h->kick_pending ? h->pend_c1 / h->pend_den : 0.0), (float)(sc->c1 / sc->den1),
-                (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1);
+                (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr);
     }
     h->kick_pending = false;
     if ((rc = check_launch(h, "kick_drift"))) return rc;
@@ -881,5 +935,189 @@ int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax) 
 }
 
 const char* pm3d_kernel_names(void) { return kKernelNames; }
+
+int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps) {
+    if (!handle || !table || nsteps < 1) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    if (!h->state_set || !h->acc_valid || !h->kick_pending)
+        return fail(h, PM3D_ESTATE, "a schedule continues a stepped state: run one pm3d_step first");
+    if (nsteps > h->tab_cap) {
+        HCHK(h, hipStreamSynchronize(h->st));
+        if (h->tab) HCHK(h, hipFree(h->tab));
+        if (h->jhist) HCHK(h, hipFree(h->jhist));
+        h->tab = nullptr;
+        h->jhist = nullptr;
+        h->tab_cap = 0;
+        int rc;
+        if ((rc = dalloc(h, &h->tab, (size_t)nsteps)) || (rc = dalloc(h, &h->jhist, (size_t)nsteps))) return rc;
+        h->tab_cap = nsteps;
+    }
+    if (!h->sched) {
+        int rc;
+        if ((rc = dalloc(h, &h->sched, 1))) return rc;
+    }
+    h->htab.resize(nsteps);
+    for (int k = 0; k < nsteps; ++k) {
+        const pm3d_sched_entry& e = table[k];
+        DevStep& d = h->htab[k];
+        d.c1p = e.c1p; d.denp = e.denp; d.c1 = e.c1; d.den1 = e.den1; d.da = e.da; d.den_d = e.den_d;
+        d.den2 = e.den2; d.s = e.s; d.sm2 = e.sm2;
+        d.kp = (float)(e.c1p / e.denp);
+        d.k1 = (float)(e.c1 / e.den1);
+        d.sd = (float)(e.da / (e.den_d * h->dx));
+        d.pad = 0.f;
+    }
+    DevSched hs{0, nsteps, 0, 0};
+    HCHK(h, hipMemcpyAsync(h->tab, h->htab.data(), nsteps * sizeof(DevStep), hipMemcpyHostToDevice, h->st));
+    HCHK(h, hipMemcpyAsync(h->sched, &hs, sizeof(hs), hipMemcpyHostToDevice, h->st));
+    HCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int pm3d_schedule_status(void* handle, int* done, double* jhist) {
+    if (!handle || !done) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    if (!h->sched) return fail(h, PM3D_ESTATE, "no schedule");
+    DevSched hs;
+    HCHK(h, hipMemcpy(&hs, h->sched, sizeof(hs), hipMemcpyDeviceToHost));
+    *done = hs.counter;
+    if (jhist && hs.counter > 0) HCHK(h, hipMemcpy(jhist, h->jhist, hs.counter * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pm3d_schedule_end(void* handle, int* done) {
+    if (!handle) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    int k = 0;
+    int rc = pm3d_schedule_status(handle, &k, nullptr);
+    if (rc) return rc;
+    if (k > 0) {  // the closing kick of the last scheduled step is pending
+        h->kick_pending = true;
+        h->pend_c1 = h->htab[k - 1].c1;
+        h->pend_den = h->htab[k - 1].den2;
+    }
+    if (done) *done = k;
+    return 0;
+}
+
+int pm3d_schedule_step(void* handle) {
+    if (!handle) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    if (!h->sched) return fail(h, PM3D_ESTATE, "no schedule");
+    return launch_sched_step(h);
+}
+
+int pm3d_batch_create(void** batch) {
+    if (!batch) return PM3D_EINVAL;
+    Batch* b = new Batch();
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        g_create_error = "batch stream/event creation failed";
+        delete b;
+        return PM3D_EHIP;
+    }
+    *batch = b;
+    return 0;
+}
+
+int pm3d_batch_destroy(void* batch) {
+    if (!batch) return PM3D_EINVAL;
+    Batch* b = (Batch*)batch;
+    hipStreamSynchronize(b->st);
+    batch_clear_graph(b);
+    for (auto e : b->joins) hipEventDestroy(e);
+    hipEventDestroy(b->fork);
+    hipStreamDestroy(b->st);
+    delete b;
+    return 0;
+}
+
+const char* pm3d_batch_last_error(void* batch) { return batch ? ((Batch*)batch)->err.c_str() : ""; }
+
+int pm3d_batch_set(void* batch, void* const* handles, int count) {
+    if (!batch || count < 0 || (count > 0 && !handles)) return PM3D_EINVAL;
+    Batch* b = (Batch*)batch;
+    hipStreamSynchronize(b->st);
+    batch_clear_graph(b);
+    b->members.clear();
+    for (int i = 0; i < count; ++i) {
+        Handle* h = (Handle*)handles[i];
+        if (!h || !h->sched) {
+            b->err = "every member needs a schedule (pm3d_set_schedule)";
+            return PM3D_ESTATE;
+        }
+        if (hipStreamSynchronize(h->st) != hipSuccess) {
+            b->err = "member stream failed";
+            return PM3D_EHIP;
+        }
+        b->members.push_back(h);
+    }
+    if (count == 0) return 0;
+    while ((int)b->joins.size() < count) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            b->err = "event creation failed";
+            return PM3D_EHIP;
+        }
+        b->joins.push_back(e);
+    }
+    // capture one step of every member as independent branches of one graph
+    if (hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        b->err = "hipStreamBeginCapture failed";
+        return PM3D_EHIP;
+    }
+    hipError_t e = hipEventRecord(b->fork, b->st);
+    int rc = 0;
+    for (int i = 0; i < count && e == hipSuccess && rc == 0; ++i) {
+        Handle* h = b->members[i];
+        const bool timing = h->timing;
+        h->timing = false;
+        e = hipStreamWaitEvent(h->st, b->fork, 0);
+        if (e == hipSuccess) rc = launch_sched_step(h);
+        if (e == hipSuccess && rc == 0) e = hipEventRecord(b->joins[i], h->st);
+        if (e == hipSuccess && rc == 0) e = hipStreamWaitEvent(b->st, b->joins[i], 0);
+        h->timing = timing;
+        if (rc) b->err = h->err;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->st, &g);
+    if (e != hipSuccess || rc || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        if (b->err.empty()) b->err = std::string("graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2);
+        return rc ? rc : PM3D_EHIP;
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        b->err = "hipGraphInstantiate failed";
+        batch_clear_graph(b);
+        return PM3D_EHIP;
+    }
+    return 0;
+}
+
+int pm3d_batch_run(void* batch, int nsteps) {
+    if (!batch || nsteps < 0) return PM3D_EINVAL;
+    Batch* b = (Batch*)batch;
+    if (!b->exec) return 0;
+    for (int k = 0; k < nsteps; ++k) {
+        hipError_t e = hipGraphLaunch(b->exec, b->st);
+        if (e != hipSuccess) {
+            b->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+            return PM3D_EHIP;
+        }
+    }
+    return 0;
+}
+
+int pm3d_batch_synchronize(void* batch) {
+    if (!batch) return PM3D_EINVAL;
+    Batch* b = (Batch*)batch;
+    hipError_t e = hipStreamSynchronize(b->st);
+    if (e != hipSuccess) {
+        b->err = std::string("hipStreamSynchronize: ") + hipGetErrorString(e);
+        return PM3D_EHIP;
+    }
+    return 0;
+}
 
 }  // extern "C"

@@ -18,6 +18,9 @@ Named workloads (periodic box L = 1, Np = N):
   peak128   N=128, mu=0.2, sigma=(0.10,0.09,0.08),             seed 1   (FP64 on one GPU)
   peak256   N=256, mu=0.1, e=0.2, p=0, axes 0.1 L via zeldovich_axes,
             + GRF P(k) ~ k^-2 for 0 < k <= k_Nyq/2, rms 0.01 mu,   seed 2   (mixed precision)
+  batch     one peak per seed (100..163): draws from default_rng(seed) in the order
+            N ~ U{32, 64, 96, 128}, mu ~ U[0.05, 0.5], e ~ U[0, 0.3], p ~ U[-e, e];
+            axes 0.1 L * zeldovich_axes(e, p)                                  (batched, mixed)
 """
 from __future__ import annotations
 
@@ -149,3 +152,14 @@ def schedule(a0: float, da_frac: float, a_end: float):
         out.append((a, da))
         a = a + da
     return out
+
+
+def batch_spec(seed: int) -> PeakSpec:
+    """One member of the batched workload (draw order: N, mu, e, p from default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    N = int(rng.choice([32, 64, 96, 128]))
+    mu = float(rng.uniform(0.05, 0.5))
+    e = float(rng.uniform(0.0, 0.3))
+    p = float(rng.uniform(-e, e))
+    ax = 0.1 * zeldovich_axes(e, p)
+    return PeakSpec(N=N, mu=mu, sigma=tuple(ax), seed=seed, name=f"batch{seed}", meta={"e": e, "p": p})

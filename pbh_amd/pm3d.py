@@ -294,11 +294,56 @@ class PM3D:
         check(self._L.pm3d_step(self._h, ctypes.byref(sc), None, None), self._h)
         a_new = a + da
         self._acc_a = a_new
+        self.last_kick = (sc.c1, sc.den2)
         return a_new, DeltaMesh(self, self._L.pm3d_deposit_serial(self._h))
 
     def synchronize(self) -> int:
         """Wait for the device; returns the count of particles flagged non-finite since the last call."""
         return check(self._L.pm3d_synchronize(self._h), self._h)
+
+    # -- device-resident schedules (graph replay; see pbh_amd.batch)
+    def schedule_entries(self, a_start: float, da_frac: float, a_end: float, max_steps: int, prev: tuple):
+        """Schedule table continuing from a_start with da = min(da_frac*a, a_end-a) (pm3d_peak.py:27);
+        prev = (c1, den2) of the last step taken.  Returns (entries, a_values) with a_values[k] the
+        scale factor after entry k."""
+        ents = (_lib.Pm3dSchedEntry * max_steps)()
+        avals = []
+        a = a_start
+        c1p, denp = prev
+        k = 0
+        while k < max_steps and a < a_end:
+            da = min(da_frac * a, a_end - a)
+            sc = step_scalars(a, da, self.smooth, self.dx, False)
+            e = ents[k]
+            e.c1p, e.denp, e.c1, e.den1, e.da, e.den_d = c1p, denp, sc.c1, sc.den1, sc.da, sc.den_d
+            e.den2, e.s, e.sm2 = sc.den2, sc.s_new, sc.sm2
+            c1p, denp = sc.c1, sc.den2
+            a = a + da
+            avals.append(a)
+            k += 1
+        return ents, avals
+
+    def set_schedule(self, entries, n: int):
+        check(self._L.pm3d_set_schedule(self._h, entries, n), self._h)
+
+    def schedule_step(self):
+        check(self._L.pm3d_schedule_step(self._h), self._h)
+
+    def schedule_status(self, with_history: bool = True):
+        done = ctypes.c_int()
+        check(self._L.pm3d_schedule_status(self._h, ctypes.byref(done), None), self._h)
+        if not with_history:
+            return done.value, None
+        hist = np.empty(max(done.value, 1))
+        check(self._L.pm3d_schedule_status(self._h, ctypes.byref(done), dptr(hist)), self._h)
+        return done.value, hist[: done.value]
+
+    def schedule_end(self, a_now: float) -> int:
+        """Hand the state back to step(): returns the number of scheduled steps taken."""
+        done = ctypes.c_int()
+        check(self._L.pm3d_schedule_end(self._h, ctypes.byref(done)), self._h)
+        self._acc_a = a_now
+        return done.value
 
     def min_jacobian(self) -> float:
         """min det(dx/dq) over the Lagrangian lattice (first shell crossing when it reaches 0)."""
