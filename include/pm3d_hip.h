@@ -119,6 +119,24 @@ int pm3d_set_timing(void* handle, int enable);
 int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax);
 const char* pm3d_kernel_names(void);  /* comma-separated */
 
+/* Initial conditions built on the device (the construction of PM3D.make_ic, pm3d.py:59-68,
+ * generalised to an explicit linear density field at a_i): delta = mu*exp(-0.5*sum (x_c/s_c)^2)
+ * around the box-centre vertex, plus optionally a Gaussian random field with sqrt(P(k)) = 1/|k| for
+ * 0 < |k| <= noise_kcut, from the SplitMix64 counter stream `seed` (Box-Muller), scaled to rms
+ * noise_rms; psi = irfftn(1j*k*dk/k2) is gathered at the particle lattice, x = (Q + psi + L/2) % L,
+ * p = coef*psi (coef = a_i^2 H(a_i)).  The handle must be FP64; a mixed handle takes the state
+ * with pm3d_copy_state. */
+typedef struct pm3d_peak_ic {
+    double mu, sx, sy, sz;        /* amplitude and Gaussian widths (box units, not fractions) */
+    double noise_rms, noise_kcut; /* random field rms (0 = none) and band edge */
+    double coef;                  /* a_i^2 H(a_i) */
+    uint64_t seed;
+} pm3d_peak_ic;
+int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic);
+/* Device-to-device copy of the state of an FP64 handle into a handle of the same N, Np and L
+ * (FP64 or mixed: positions become cell + offset, momenta FP32). */
+int pm3d_copy_state(void* dst_handle, void* src_handle);
+
 /* ---------------------------------------------------------------------------------------------
  * Device-resident schedules and batched graph replay.  No counterpart in the reference (its driver
  * loops over PM3D.step from Python, pbhgr/scripts/pm3d_peak.py:26-33); these entry points run the

@@ -32,6 +32,11 @@ class Pm3dStepScalars(ctypes.Structure):
                 ("sm2", c_double), ("need_forces", c_int32), ("pad", c_int32)]
 
 
+class Pm3dPeakIC(ctypes.Structure):
+    _fields_ = [("mu", c_double), ("sx", c_double), ("sy", c_double), ("sz", c_double),
+                ("noise_rms", c_double), ("noise_kcut", c_double), ("coef", c_double), ("seed", ctypes.c_uint64)]
+
+
 class Pm3dSchedEntry(ctypes.Structure):
     _fields_ = [("c1p", c_double), ("denp", c_double), ("c1", c_double), ("den1", c_double),
                 ("da", c_double), ("den_d", c_double), ("den2", c_double), ("s", c_double), ("sm2", c_double)]
@@ -64,6 +69,8 @@ SIGNATURES = {
     "pm3d_set_timing": (c_int, [c_void_p, c_int]),
     "pm3d_get_timings": (c_int, [c_void_p, PI64, PD, c_int]),
     "pm3d_kernel_names": (c_char_p, []),
+    "pm3d_make_peak_ic": (c_int, [c_void_p, POINTER(Pm3dPeakIC)]),
+    "pm3d_copy_state": (c_int, [c_void_p, c_void_p]),
     "pm3d_set_schedule": (c_int, [c_void_p, POINTER(Pm3dSchedEntry), c_int]),
     "pm3d_schedule_step": (c_int, [c_void_p]),
     "pm3d_schedule_status": (c_int, [c_void_p, POINTER(c_int), PD]),

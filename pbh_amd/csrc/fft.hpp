@@ -264,7 +264,47 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
         A[j * PP + c] = src[off(j, c)];
     }
     __syncthreads();
-    if constexpr (MODE < 2) {
+    if constexpr (MODE == 3) {
+        // Zel'dovich displacement: psi_c = 1j*k_c*dk/k2 with dk(0) = 0, evaluated as numpy does
+        // ((0 + 1j*k_c) * dk, then complex/real as a multiply by 1/k2), then inverse x FFTs
+        const cplx<R>* res = lds_fft<R, N, P, false, 1>(A, B, tw);
+        cplx<R> dkr[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int t = threadIdx.x + e * kThreads;
+            dkr[e] = cplx<R>{(R)0, (R)0};
+            if (t < N * P) dkr[e] = res[(t / P) * PP + t % P];
+        }
+        __syncthreads();
+        for (int comp = 0; comp < 3; ++comp) {
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                const int t = threadIdx.x + e * kThreads;
+                if (t < N * P) {
+                    const int c = t % P, j = t / P;
+                    const int iy = main ? y : col0 + c, iz = main ? col0 + c : NH;
+                    const double kx = kfull(j, N, kval), ky = kfull(iy, N, kval), kz = khalf(iz, kval);
+                    const bool dc = (j == 0) && (iy == 0) && (iz == 0);
+                    double k2 = (kx * kx + ky * ky) + kz * kz;
+                    if (dc) k2 = 1.0;
+                    const double rk = 1.0 / k2;
+                    const double kc = comp == 0 ? kx : comp == 1 ? ky : kz;
+                    double pr = (-(kc * (double)dkr[e].im)) * rk, pi = (kc * (double)dkr[e].re) * rk;
+                    if (dc) { pr = 0.0; pi = 0.0; }
+                    A[j * PP + c] = cplx<R>{(R)pr, (R)pi};
+                }
+            }
+            __syncthreads();
+            const cplx<R>* r2 = lds_fft<R, N, P, true, 1>(A, B, tw);
+            cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
+            for (int t = threadIdx.x; t < N * P; t += kThreads) {
+                const int c = t % P, j = t / P;
+                dst[off(j, c)] = r2[j * PP + c];
+            }
+            __syncthreads();
+        }
+        return;
+    } else if constexpr (MODE < 2) {
         const cplx<R>* res = lds_fft<R, N, P, MODE == 1, 1>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;

@@ -25,6 +25,7 @@
 #include "particles.hpp"
 #include "particles_mixed.hpp"
 #include "deposit.hpp"
+#include "ic.hpp"
 
 using namespace pm3d;
 
@@ -201,9 +202,12 @@ struct Fft {
         else if (mode == 1)
             k_x_pass<R, N, 1><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0,
                                                          nullptr, nullptr);
-        else
+        else if (mode == 2)
             k_x_pass<R, N, 2><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s, sm2,
                                                          sm2 > 0.0 ? 1 : 0, sch, h->tab);
+        else
+            k_x_pass<R, N, 3><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, 0, 0, 0,
+                                                         nullptr, nullptr);
     }
 
     // inverse y (ncomp spectra from base) + c2r z into out (ncomp interleaved)
@@ -485,6 +489,16 @@ int poisson_impl(Handle* h, const double* delta, double s, double sm2, double* g
     HCHK(h, hipStreamSynchronize(h->st));
     for (long long i = 0; i < 3 * h->M; ++i) g[i] = (double)ho[i];
     return 0;
+}
+
+// spectrum amplitude filter for the random field (FP64 handles only)
+template <int N>
+void spec_amp_n(Handle* h, double kcut) {
+    using R = double;
+    const size_t specN = (size_t)N * N * (N / 2);
+    cplx<R>* sp = (cplx<R>*)h->spec;
+    k_spec_amp<R><<<grid_cap((long long)specN + (long long)N * N), kThreads, 0, h->st>>>(sp, sp + specN, N,
+                                                                                       1.0 / ((double)N * h->dx), kcut);
 }
 
 // One step driven by the device schedule (for graph capture): kick+drift with the scheduled
@@ -935,6 +949,80 @@ int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax) 
 }
 
 const char* pm3d_kernel_names(void) { return kKernelNames; }
+
+int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic) {
+    if (!handle || !ic) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    if (h->mixed) return fail(h, PM3D_EUNSUPPORTED, "initial conditions are built on an FP64 handle (copy with pm3d_copy_state)");
+    const long long M = h->M;
+    const int nb = (int)grid_cap(M);
+    int rc = ensure_sbuf(h, (2 * M + 2 * (size_t)nb) * sizeof(double));
+    if (rc) return rc;
+    double* field = h->sbuf;
+    double* noise = h->sbuf + M;
+    double* mom = h->sbuf + 2 * M;
+    k_peak_field<<<nb, kThreads, 0, h->st>>>(field, h->N, h->dx, h->L, ic->mu, ic->sx, ic->sy, ic->sz);
+    if (ic->noise_rms > 0.0) {
+        // white noise -> rfftn -> * |k|^-1 (0 < |k| <= kcut) -> irfftn -> scaled to rms noise_rms
+        k_white<<<nb, kThreads, 0, h->st>>>(noise, M, splitmix64(ic->seed));
+        if ((rc = fft_forward(h, noise, false)) || (rc = fft_xpass(h, 0, 0, 0))) return rc;
+        switch (h->N) {
+            case 8: spec_amp_n<8>(h, ic->noise_kcut); break;
+            case 16: spec_amp_n<16>(h, ic->noise_kcut); break;
+            case 32: spec_amp_n<32>(h, ic->noise_kcut); break;
+            case 48: spec_amp_n<48>(h, ic->noise_kcut); break;
+            case 64: spec_amp_n<64>(h, ic->noise_kcut); break;
+            case 96: spec_amp_n<96>(h, ic->noise_kcut); break;
+            case 128: spec_amp_n<128>(h, ic->noise_kcut); break;
+            case 256: spec_amp_n<256>(h, ic->noise_kcut); break;
+            case 512: spec_amp_n<512>(h, ic->noise_kcut); break;
+        }
+        if ((rc = fft_xpass(h, 1, 0, 0)) || (rc = fft_inverse(h, h->spec, 1, noise))) return rc;
+        k_moments<<<nb, kThreads, 0, h->st>>>(noise, M, mom);
+        std::vector<double> hm(2 * nb);
+        HCHK(h, hipMemcpyAsync(hm.data(), mom, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HCHK(h, hipStreamSynchronize(h->st));
+        double s1 = 0.0, s2 = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            s1 += hm[2 * b];
+            s2 += hm[2 * b + 1];
+        }
+        const double mean = s1 / M, var = s2 / M - mean * mean;
+        if (!(var > 0.0)) return fail(h, PM3D_EINVAL, "random field has zero variance (empty band)");
+        k_axpy<<<nb, kThreads, 0, h->st>>>(field, noise, M, ic->noise_rms / std::sqrt(var));
+    }
+    // Zel'dovich displacement psi = irfftn(1j*k*dk/k2) on the mesh, then the lattice placement
+    if ((rc = fft_forward(h, field, false)) || (rc = fft_xpass(h, 3, 0, 0)) || (rc = fft_inverse(h, h->gspec, 3, h->grad)))
+        return rc;
+    k_lattice<<<grid_for(h->n), kThreads, 0, h->st>>>((const double*)h->grad, h->N, h->Np, h->L, ic->coef, h->x, h->p);
+    HCHK(h, hipStreamSynchronize(h->st));
+    h->state_set = true;
+    h->acc_valid = false;
+    h->kick_pending = false;
+    return check_launch(h, "make_peak_ic");
+}
+
+int pm3d_copy_state(void* dst_handle, void* src_handle) {
+    if (!dst_handle || !src_handle) return PM3D_EINVAL;
+    Handle* d = (Handle*)dst_handle;
+    Handle* s = (Handle*)src_handle;
+    if (s->mixed || !s->state_set) return fail(d, PM3D_EINVAL, "source must be an FP64 handle with a state");
+    if (s->n != d->n || s->N != d->N || s->L != d->L) return fail(d, PM3D_EINVAL, "source and destination differ in N, Np or L");
+    int rc = 0;
+    HCHK(d, hipStreamSynchronize(s->st));
+    if (!d->mixed) {
+        HCHK(d, hipMemcpyAsync(d->x, s->x, 3 * s->n * sizeof(double), hipMemcpyDeviceToDevice, d->st));
+        HCHK(d, hipMemcpyAsync(d->p, s->p, 3 * s->n * sizeof(double), hipMemcpyDeviceToDevice, d->st));
+    } else {
+        k_soa_to_mx<<<grid_for(d->n), kThreads, 0, d->st>>>(s->x, s->p, d->n, d->N, d->dx, d->cell, d->off, d->pf);
+    }
+    HCHK(d, hipStreamSynchronize(d->st));
+    d->state_set = true;
+    d->acc_valid = false;
+    d->kick_pending = false;
+    (void)rc;
+    return check_launch(d, "copy_state");
+}
 
 int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps) {
     if (!handle || !table || nsteps < 1) return PM3D_EINVAL;

@@ -9,7 +9,9 @@ factor a_i in EdS:
     psi = -grad lap^-1 delta_i  (psi_k = i k delta_k / k^2)          displacement at a_i
     x = (q + CIC(psi)(q) + L/2) % L,   p = a_i^2 H(a_i) psi,   H(a) = H_I a^-3/2
 
-``mu`` is the linear density contrast at a_i (not extrapolated to a = 1).  The peak is centred on
+``mu`` is the linear density contrast at a_i (not extrapolated to a = 1).  Random fields come from
+the SplitMix64 counter stream of `white_noise`, which the device generator reproduces cell by cell,
+so `device_ic` builds the same initial data on the GPU.  The peak is centred on
 the box-centre mesh vertex, as in pm3d.py:42,66, and particles sit on the lattice
 q = (i + 1/2) L/Np - L/2 in C order (pm3d.py:64-65).  Oracle and device receive the same (x, p).
 
@@ -17,7 +19,7 @@ Named workloads (periodic box L = 1, Np = N):
   peak64    N=64,  mu=0.3, sigma=(0.12,0.11,0.10),             seed 0   (CPU reference case)
   peak128   N=128, mu=0.2, sigma=(0.10,0.09,0.08),             seed 1   (FP64 on one GPU)
   peak256   N=256, mu=0.1, e=0.2, p=0, axes 0.1 L via zeldovich_axes,
-            + GRF P(k) ~ k^-2 for 0 < k <= k_Nyq/2, rms 0.01 mu,   seed 2   (mixed precision)
+            + GRF P(k) ~ k^-2 for 0 < k <= k_Nyq/2, rms 0.01 mu, noise stream seed 2  (mixed)
   batch     one peak per seed (100..163): draws from default_rng(seed) in the order
             N ~ U{32, 64, 96, 128}, mu ~ U[0.05, 0.5], e ~ U[0, 0.3], p ~ U[-e, e];
             axes 0.1 L * zeldovich_axes(e, p)                                  (batched, mixed)
@@ -76,6 +78,32 @@ def _k_grids(N: int, L: float):
     return kx, ky, kz, k2
 
 
+_U64 = np.uint64
+
+
+def splitmix64(z):
+    """SplitMix64 finaliser (Steele, Lea & Flood, OOPSLA 2014) on uint64 arrays (wrapping arithmetic);
+    the same stream as the device generator in csrc/ic.hpp."""
+    z = np.asarray(z, dtype=np.uint64) + _U64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> _U64(30))) * _U64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> _U64(27))) * _U64(0x94D049BB133111EB)
+    return z ^ (z >> _U64(31))
+
+
+def white_noise(seed: int, shape) -> np.ndarray:
+    """Standard normal field from the counter stream `seed`: Box-Muller of two 53-bit uniforms per
+    cell, u1 in (0, 1], u2 in [0, 1).  Cell i of the flattened field is independent of the shape."""
+    M = int(np.prod(shape))
+    key = splitmix64(np.array([seed], dtype=np.uint64))[0]
+    i = np.arange(M, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        a = splitmix64(key + _U64(2) * i)
+        b = splitmix64(key + _U64(2) * i + _U64(1))
+    u1 = ((a >> _U64(11)) + _U64(1)).astype(np.float64) * 2.0 ** -53
+    u2 = (b >> _U64(11)).astype(np.float64) * 2.0 ** -53
+    return (np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2)).reshape(shape)
+
+
 def linear_delta(spec: PeakSpec) -> np.ndarray:
     """delta_i on the mesh (cell centres relative to the box-centre vertex)."""
     N, L = spec.N, spec.L
@@ -85,8 +113,7 @@ def linear_delta(spec: PeakSpec) -> np.ndarray:
     sx, sy, sz = (s * L for s in spec.sigma)
     delta = spec.mu * np.exp(-0.5 * ((X / sx) ** 2 + (Y / sy) ** 2 + (Z / sz) ** 2))
     if spec.noise_rms_frac > 0:
-        rng = np.random.default_rng(spec.seed)
-        white = np.fft.rfftn(rng.standard_normal((N, N, N)))
+        white = np.fft.rfftn(white_noise(spec.seed, (N, N, N)))
         kx, ky, kz, k2 = _k_grids(N, L)
         kk = np.sqrt(k2)
         kk[0, 0, 0] = 0.0
@@ -163,3 +190,30 @@ def batch_spec(seed: int) -> PeakSpec:
     p = float(rng.uniform(-e, e))
     ax = 0.1 * zeldovich_axes(e, p)
     return PeakSpec(N=N, mu=mu, sigma=tuple(ax), seed=seed, name=f"batch{seed}", meta={"e": e, "p": p})
+
+
+def device_ic(pm, spec: PeakSpec, engine=None):
+    """Build the workload's initial state on the GPU (pm3d_make_peak_ic) in `pm`.
+
+    An FP64 PM3D builds it in place; a mixed PM3D receives it from an FP64 `engine` handle of the
+    same N (created if not given; reuse one across requests to avoid re-allocation)."""
+    import ctypes
+    from . import _lib
+    from .pm3d import PM3D
+    if spec.n_side != spec.N:
+        raise ValueError("device initial conditions need Np == N")
+    ic = _lib.Pm3dPeakIC()
+    ic.mu = spec.mu
+    ic.sx, ic.sy, ic.sz = (s * spec.L for s in spec.sigma)
+    ic.noise_rms = spec.noise_rms_frac * spec.mu if spec.noise_rms_frac > 0 else 0.0
+    ic.noise_kcut = spec.noise_kmax_frac * np.pi * spec.N / spec.L
+    ic.coef = spec.a_i ** 2 * hubble(spec.a_i)
+    ic.seed = spec.seed
+    if pm.precision == "fp64":
+        _lib.check(pm._L.pm3d_make_peak_ic(pm._h, ctypes.byref(ic)), pm._h)
+    else:
+        eng = engine if engine is not None else PM3D(spec.N, spec.L, spec.N, precision="fp64")
+        _lib.check(eng._L.pm3d_make_peak_ic(eng._h, ctypes.byref(ic)), eng._h)
+        _lib.check(pm._L.pm3d_copy_state(pm._h, eng._h), pm._h)
+    pm._acc_a = None
+    return pm

@@ -120,3 +120,20 @@ def test_batched_collapse_equals_plain():
         assert x.crossed and y.crossed
         assert x.steps == y.steps
         assert x.t_c == y.t_c
+
+
+@pytest.mark.parametrize("N,noise", [(32, 0.01), (64, 0.0), (96, 0.02)])
+def test_device_ic_matches_host(N, noise):
+    """pm3d_make_peak_ic (field, random stream, FFT displacement, lattice) == pbh_amd.ic.make."""
+    from pbh_amd import ic
+    from pbh_amd.pm3d import PM3D
+    say("device ic", N, noise)
+    spec = ic.PeakSpec(N=N, mu=0.2, sigma=(0.1, 0.09, 0.08), seed=5, noise_rms_frac=noise)
+    st = ic.make(spec)
+    pm = ic.device_ic(PM3D(N, 1.0, N), spec)
+    d = (pm.x - st["x"] + 0.5) % 1.0 - 0.5
+    assert np.abs(d).max() <= 1e-12
+    assert rel(pm.p, st["p"]) <= 1e-11
+    mx = ic.device_ic(PM3D(N, 1.0, N, precision="mixed"), spec)
+    d = (mx.x - st["x"] + 0.5) % 1.0 - 0.5
+    assert np.abs(d).max() <= 1.0 / N * 2 ** -23
