@@ -32,16 +32,18 @@ PEAK_HBM_GBS = 8000.0
 
 def algorithmic_bytes(N: int, n: int, prec: str) -> dict:
     """Bytes each kernel must move once per step (one read of every input, one write of every
-    output), FP64 layout: positions/momenta/accelerations 3 x 8 B SoA, int64 deposit mesh, complex128
-    half spectra (Mh = N*N*(N/2+1) points), interleaved 3 x 8 B gradient mesh."""
+    output).  FP64: positions/momenta/accelerations 3 x 8 B SoA, 16 B fixed-point cells, complex128
+    half spectra (Mh = N*N*(N/2+1) points), interleaved 3 x 8 B gradient mesh.  Mixed: positions
+    3 x (4 B cell + 4 B offset), FP32 momenta/accelerations/spectra/gradients, 8 B cells."""
     M = N ** 3
     Mh = N * N * (N // 2 + 1)
     r = 8 if prec == "fp64" else 4
     c = 2 * r
+    q = 16 if prec == "fp64" else 8                    # fixed-point cell: 128-bit (FP64) / 64-bit (mixed)
     return {
         "kick_drift": 5 * 3 * r * n,                   # read x, p, acc; write x, p
-        "deposit": 3 * r * n + 8 * M,                  # read x; write int64 mesh
-        "r2c_z": 8 * M + c * Mh,                       # read mesh; write spectrum
+        "deposit": 24 * n + q * M,                     # read x (24 B both modes); write the mesh
+        "r2c_z": q * M + c * Mh,                       # read mesh; write spectrum
         "y_fwd": 2 * c * Mh,
         "x_poisson": c * Mh + 3 * c * Mh,              # read 1, write 3 spectra
         "y_inv": 2 * 3 * c * Mh,

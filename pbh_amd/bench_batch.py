@@ -48,7 +48,16 @@ def batched_vs_plain(seeds=range(100, 164), precision="mixed", poll=8):
     same = all(a.crossed == b.crossed and a.steps == b.steps and (a.t_c == b.t_c or not a.crossed)
                for a, b in zip(rp, rb))
     n = len(seeds)
+    # plain FP64 one-at-a-time (the reference-precision GPU path)
+    p64, engines = _members(seeds, "fp64", engines=engines)
+    t0 = time.perf_counter()
+    r64 = [run_to_crossing(m.pm, m.a0, m.da_frac, m.a_max) for m in p64]
+    t_plain64 = time.perf_counter() - t0
+    del p64
+    dtc = max(abs(a.t_c - b.t_c) / a.t_c for a, b in zip(r64, rb) if a.crossed and b.crossed)
     return {
+        "plain_fp64_s": t_plain64, "plain_fp64_solves_per_s": n / t_plain64,
+        "speedup_vs_plain_fp64": t_plain64 / t_batch, "max_rel_t_c_mixed_vs_fp64": dtc,
         "members": n, "precision": precision, "poll_steps": poll,
         "plain_s": t_plain, "batched_s": t_batch,
         "plain_solves_per_s": n / t_plain, "batched_solves_per_s": n / t_batch,

@@ -258,3 +258,32 @@ def test_tiled_deposit_equals_atomic(PM3D):
     d_step = np.asarray(dm)
     rho = pm.deposit(pm.x)
     assert np.array_equal(d_step, rho / 1.0 - 1.0)
+
+
+def test_driver_rows_match_oracle_loop(tmp_path):
+    """The drop-in driver (16^3 mesh, 8^3 particles, to t = 3 t_H) against the same loop on the oracle."""
+    import json
+    from pbh_amd.ltb import LTBGaussian, profile_table
+    from pbh_amd.pm3d_peak import main
+    say("driver")
+    out = main(["--N", "16", "--Np", "8", "--tend", "3", "--mu", "0.3", "--e", "0.1", "--out", str(tmp_path)])
+    ref_ltb = LTBGaussian(0.3)
+    rm, t_H = ref_ltb.r_m, ref_ltb.t_H
+    pm = O.OraclePM3D(16, 8.0 * rm, 8, seed=0, smooth=2.0).make_ic(0.3, profile_table("yoo")["spline"], 0.1, 0.0, 0.0, (2.0, 10.0))
+    a, step, next_diag, rows = 1.0, 0, 0.0, []
+    a_end = (3.0 * t_H / O.T_I) ** (2.0 / 3.0)
+    while a < a_end:
+        a, delta = pm.step(a, min(0.02 * a, a_end - a))
+        step += 1
+        t = O.T_I * a ** 1.5
+        if t / t_H >= next_diag or a >= a_end:
+            d = O.analyse(pm, a)
+            d.update(step=step, delta_max=float(delta.max()))
+            rows.append(d)
+            next_diag = t / t_H + max(0.5, 0.02 * ref_ltb.tC(1e-6) / t_H)
+    assert len(rows) == len(out["rows"])
+    for r_ref, r_gpu in zip(rows, out["rows"]):
+        assert r_gpu["step"] == r_ref["step"]
+        for key in ("axes_com_0.2", "hoop_0.1", "m_in_2c", "rho_max_over_mean", "delta_max"):
+            np.testing.assert_allclose(np.asarray(r_gpu[key]), np.asarray(r_ref[key]), rtol=1e-8, err_msg=key)
+    assert (tmp_path / "mu0.3_e0.1_p0_N16_Np8_L8_sm2.json").exists()

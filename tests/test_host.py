@@ -62,3 +62,14 @@ def test_schedule_matches_driver_loop():
         assert da == min(0.02 * a, 1.5 - a)
         a = a + da
     assert a >= 1.5
+
+
+def test_driver_flags_and_tag_match_reference_driver():
+    """pbh_amd.pm3d_peak keeps the flags, defaults and output tag of pm3d_peak.py:8-13,21."""
+    from pbh_amd.pm3d_peak import parser, run_tag
+    a = parser().parse_args([])
+    assert (a.mu, a.e, a.p, a.N, a.Np, a.L, a.tend, a.tail, a.k1, a.k2, a.profile, a.seed, a.smooth, a.da_frac,
+            a.out) == (0.1, 0.0, 0.0, 128, 64, 8.0, 40.0, 0.0, 2.0, 10.0, "yoo", 0, 2.0, 0.02, "runs/pm3d")
+    assert run_tag(a) == "mu0.1_e0_p0_N128_Np64_L8_sm2"
+    b = parser().parse_args(["--tail", "0.05", "--seed", "3", "--mu", "0.25"])
+    assert run_tag(b) == "mu0.25_e0_p0_N128_Np64_L8_sm2_tail0.05_k2-10_s3"
