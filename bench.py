@@ -4,7 +4,7 @@
 
 Workload (default "peak128"): periodic box L=1, 128^3 mesh, 128^3 = 2,097,152 particles, one
 triaxial Gaussian peak mu=0.2, sigma=(0.10,0.09,0.08) L, Zel'dovich growing mode at a_i=0.01 in EdS,
-seed 1, FP64 (positions, momenta, FFT, forces; int64 fixed-point deposit).  A step is one KDK
+seed 1, FP64 (positions, momenta, FFT, forces; 128-bit fixed-point deposit at 2^-52).  A step is one KDK
 leapfrog step of PM3D.step (deposit, FFT Poisson, gather, kick, drift) with inputs resident in HBM.
 
 Prints one JSON line: value = particle-steps/s = n*K/wall over K timed steps; `roofline` for the
@@ -34,7 +34,9 @@ def algorithmic_bytes(N: int, n: int, prec: str) -> dict:
     """Bytes each kernel must move once per step (one read of every input, one write of every
     output).  FP64: positions/momenta/accelerations 3 x 8 B SoA, 16 B fixed-point cells, complex128
     half spectra (Mh = N*N*(N/2+1) points), interleaved 3 x 8 B gradient mesh.  Mixed: positions
-    3 x (4 B cell + 4 B offset), FP32 momenta/accelerations/spectra/gradients, 8 B cells."""
+    3 x (4 B cell + 4 B offset), FP32 momenta/accelerations/spectra/gradients, 8 B cells.  The mixed
+    drift rewrites a particle's cell only when it moves to another cell, so kick_drift counts the
+    offsets (3 x 4 B) as its position traffic."""
     M = N ** 3
     Mh = N * N * (N // 2 + 1)
     r = 8 if prec == "fp64" else 4
@@ -48,7 +50,7 @@ def algorithmic_bytes(N: int, n: int, prec: str) -> dict:
         "x_poisson": c * Mh + 3 * c * Mh,              # read 1, write 3 spectra
         "y_inv": 2 * 3 * c * Mh,
         "c2r_z": 3 * c * Mh + 3 * r * M,               # read 3 spectra; write interleaved gradient
-        "gather": 3 * r * n + 3 * r * M + 3 * r * n,   # read x, gradient; write acc
+        "gather": 24 * n + 3 * r * M + 3 * r * n,      # read x (24 B both modes), gradient; write acc
     }
 
 
@@ -80,6 +82,28 @@ def cpu_baseline(st: dict, spec, nsteps: int, da_frac: float) -> dict:
                       f"{os.cpu_count()} CPUs, affinity {len(os.sched_getaffinity(0))})"}
 
 
+def copy_probe(mib: int = 512, reps: int = 20) -> dict:
+    """Plain device-to-device copy of `mib` MiB (torch's copy kernel), timed with events: the
+    attainable HBM rate on this box, and the kernel whose FETCH_SIZE / WRITE_SIZE counters scale
+    the traffic figures."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * n * 8
+    return {"copy_probe": {"read_bytes": n * 8, "write_bytes": n * 8, "avg_us": ms * 1e3,
+                           "gbs": nbytes / (ms * 1e-3) / 1e9}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,7 +114,11 @@ def main():
     ap.add_argument("--da-frac", type=float, default=0.0125)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-probe", action="store_true", help="time a plain 512 MiB device copy and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        print(json.dumps(copy_probe()), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no multi-GPU path",
               file=sys.stderr)
@@ -161,6 +189,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "step_kernel_sum_us": tot / args.steps * 1e3,
+        "fft_tiles": pm.fft_tiles(),
     }
     print(json.dumps(out), flush=True)
 

@@ -118,6 +118,13 @@ int pm3d_min_jacobian(void* handle, double* jmin);
 int pm3d_set_timing(void* handle, int enable);
 int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax);
 const char* pm3d_kernel_names(void);  /* comma-separated */
+/* FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z), two bits: bit 0 = 32 KB rather than
+ * 16 KB LDS buffers (more pencils per workgroup; same bits), bit 1 = radix 8 (FP64) / 16 (FP32)
+ * rather than 4 (a different rounding sequence).  pm3d_create times both buffer sizes and keeps
+ * the faster (cached per mesh size and precision); the radix is fixed per pass.  Environment
+ * PM3D_FFT_TILES = one digit 0-3, or five digits, forces the variants.  Returns the number of
+ * passes.  No reference counterpart (numpy's FFT has no tiles). */
+int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax);
 
 /* Initial conditions built on the device (the construction of PM3D.make_ic, pm3d.py:59-68,
  * generalised to an explicit linear density field at a_i): delta = mu*exp(-0.5*sum (x_c/s_c)^2)

@@ -177,11 +177,17 @@ __device__ __forceinline__ double block_min(double v) {
 // Cell accumulators of the deposit.  Integer addition commutes, so every launch order gives the
 // same bits.
 //   narrow: one uint64 word per cell, value = q * 2^-FB
-//   wide  : two uint64 words {lo, hi} per cell, value = (hi*2^64 + lo) * 2^-60; the carry out of lo
+//   wide  : two uint64 words {lo, hi} per cell, value = (hi*2^64 + lo) * 2^-52; the carry out of lo
 //           is detected from the value atomicAdd returns, and the number of carries equals
-//           floor(sum/2^64) whatever the order.  Quantum 2^-60 ~ 8.7e-19 of a particle, so the FP64
-//           mode is exact to FP64 rounding even where delta = rho - 1 is tiny.
-constexpr int kWideFB = 60;
+//           floor(sum/2^64) whatever the order.  Each corner weight is rounded to 2^-52 (the spacing
+//           of doubles just below 1), so the FP64 mode stays at FP64 rounding level.
+constexpr int kWideFB = 52;
+
+// Round-to-nearest-even of w*2^52 as an integer, for w in [0, 1]: the FMA lands in [2^52, 2^53],
+// where doubles are the integers, and the bit pattern minus that of 2^52 is the integer.
+__device__ __forceinline__ unsigned long long q52(double w) {
+    return (unsigned long long)__double_as_longlong(fma(w, 0x1p52, 0x1p52)) - 0x4330000000000000ull;
+}
 
 template <bool WIDE>
 __device__ __forceinline__ void fx_atomic_add(unsigned long long* mesh, size_t cell, unsigned long long q) {
@@ -198,7 +204,7 @@ template <bool WIDE>
 __device__ __forceinline__ double fx_value(const unsigned long long* mesh, size_t cell, double qscale) {
     if constexpr (WIDE) {
         const ulonglong2 v = *(const ulonglong2*)(mesh + 2 * cell);
-        return (double)v.y * 16.0 + (double)v.x * 0x1p-60;
+        return (double)v.y * 4096.0 + (double)v.x * 0x1p-52;
     } else {
         return (double)mesh[cell] * qscale;
     }

@@ -10,8 +10,9 @@
 // workgroup that owns its site, in that workgroup's own segment of `far` (no global counter), and is
 // added afterwards by k_deposit_far_seg with global atomics.
 //
-// Integer accumulation commutes, so the result is bitwise identical for any schedule, and
-// identical to the reference-shaped atomic deposit.
+// Each corner weight ((wx*wy)*wz, the reference's order) is rounded to the fixed-point grid on its
+// own, so every path that deposits a particle adds the same integers; integer accumulation
+// commutes, so the result is bitwise identical for any schedule and to the atomic deposit.
 #pragma once
 #include "common.hpp"
 #include "particles.hpp"
@@ -20,27 +21,29 @@
 namespace pm3d {
 
 // Particle sources: raw loads of particle i (issued in batches, so that several sites' loads are in
-// flight per thread), then base cell, upper cell and the 8 fixed-point weights.
+// flight per thread), the CIC base/upper cells with the per-axis weights, and the fixed-point value
+// of one corner weight (wx*wy)*wz, evaluated in the reference's order (pm3d.py:85-88).
 struct SrcF64 {
     const double* x;
     long long n;
     double dx, qs;
     long long one;
     int N;
+    using W = double;
     struct Raw {
         double v[3];
     };
     __device__ __forceinline__ Raw load(long long i) const { return Raw{{x[i], x[n + i], x[2 * n + i]}}; }
-    __device__ __forceinline__ void stencil(const Raw& r, int lo[3], int hi[3], unsigned long long q[8]) const {
-        double f[3], w[8];
-        long long qq[8];
+    __device__ __forceinline__ void axes(const Raw& r, int lo[3], int hi[3], double w0[3], double w1[3]) const {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) cic_axis_fast(r.v[d], dx, N, lo[d], hi[d], f[d]);
-        cic_weights(f, w);
-        quantise8(f, w, qs, one, qq);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) q[c] = (unsigned long long)qq[c];
+        for (int d = 0; d < 3; ++d) {
+            double f;
+            cic_axis_fast(r.v[d], dx, N, lo[d], hi[d], f);
+            w0[d] = 1.0 - f;
+            w1[d] = f;
+        }
     }
+    __device__ __forceinline__ unsigned long long quant(double w) const { return q52(w); }
 };
 
 struct SrcMx {
@@ -48,6 +51,7 @@ struct SrcMx {
     const float* off;
     long long n;
     int N;
+    using W = float;
     struct Raw {
         int c[3];
         float f[3];
@@ -55,16 +59,16 @@ struct SrcMx {
     __device__ __forceinline__ Raw load(long long i) const {
         return Raw{{cell[i], cell[n + i], cell[2 * n + i]}, {off[i], off[n + i], off[2 * n + i]}};
     }
-    __device__ __forceinline__ void stencil(const Raw& r, int lo[3], int hi[3], unsigned long long q[8]) const {
-        float w[8];
+    __device__ __forceinline__ void axes(const Raw& r, int lo[3], int hi[3], float w0[3], float w1[3]) const {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             lo[d] = r.c[d];
             hi[d] = lo[d] + 1 == N ? 0 : lo[d] + 1;
+            w0[d] = 1.0f - r.f[d];
+            w1[d] = r.f[d];
         }
-        mx_weights(r.f, w);
-        mx_quantise8(r.f, w, q);
     }
+    __device__ __forceinline__ unsigned long long quant(float w) const { return q31(w); }
 };
 
 // Displacement of the base cell from the site along one axis (minimum image).
@@ -80,12 +84,14 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
                                                     int* __restrict__ far, unsigned int* __restrict__ farcnt) {
     constexpr int NCELL = BX * BY * N;
     constexpr int EX = BX + 2, EY = BY + 2, NSITE = EX * EY * N;
+    // WIDE: the 128-bit cell is accumulated as two 64-bit sums of the low and high 32-bit halves of
+    // each contribution (no return value, no carry test per add) and recombined exactly at the end
     __shared__ unsigned long long acc[NCELL];
-    __shared__ unsigned int carry[WIDE ? NCELL : 1];
+    __shared__ unsigned long long acch[WIDE ? NCELL : 1];
     __shared__ unsigned int nfar;
     for (int t = threadIdx.x; t < NCELL; t += NT) {
         acc[t] = 0ull;
-        if (WIDE) carry[t] = 0u;
+        if (WIDE) acch[t] = 0ull;
     }
     if (threadIdx.x == 0) nfar = 0u;
     __syncthreads();
@@ -112,8 +118,8 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
         const int sx = wrapN(x0 + ex - 1, N), sy = wrapN(y0 + ey - 1, N);
         const long long i = ((long long)sx * N + sy) * N + sz;
         int lo[3], hi[3];
-        unsigned long long q[8];
-        src.stencil(raw[u], lo, hi, q);
+        typename Src::W w0[3], w1[3];
+        src.axes(raw[u], lo, hi, w0, w1);
         const int ox = site_offset(lo[0], sx, N), oy = site_offset(lo[1], sy, N), oz = site_offset(lo[2], sz, N);
         const bool is_far = ox < -1 || ox > 0 || oy < -1 || oy > 0 || oz < -1 || oz > 0;
         const bool own = ex >= 1 && ex <= BX && ey >= 1 && ey <= BY;
@@ -134,15 +140,20 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
 #pragma unroll
         for (int cxy = 0; cxy < 4; ++cxy) {
             const int cx = cxy >> 1, cy = cxy & 1;
-            if (rx[cx] < BX && ry[cy] < BY) {
+            if (rx[cx] < BX && ry[cy] < BY) {  // only the corners that land in this column
                 const int col = (rx[cx] * BY + ry[cy]) * N;
+                const typename Src::W wxy = (cx ? w1[0] : w0[0]) * (cy ? w1[1] : w0[1]);
 #pragma unroll
                 for (int cz = 0; cz < 2; ++cz) {
-                    const unsigned long long v = q[cxy * 2 + cz];
+                    const unsigned long long v = src.quant(wxy * (cz ? w1[2] : w0[2]));
                     if (v != 0ull) {
                         const int cell = col + (cz ? hi[2] : lo[2]);
-                        const unsigned long long old = atomicAdd(&acc[cell], v);
-                        if (WIDE && old + v < old) atomicAdd(&carry[cell], 1u);
+                        if (WIDE) {
+                            atomicAdd(&acc[cell], v & 0xffffffffull);
+                            atomicAdd(&acch[cell], v >> 32);
+                        } else {
+                            atomicAdd(&acc[cell], v);
+                        }
                     }
                 }
             }
@@ -154,29 +165,71 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
     for (int t = threadIdx.x; t < NCELL; t += NT) {
         const int rz = t % N, ry = (t / N) % BY, rx = t / (N * BY);
         const size_t cell = ((size_t)(x0 + rx) * N + (y0 + ry)) * N + rz;
-        if (WIDE)
-            *(ulonglong2*)(mesh + 2 * cell) = make_ulonglong2(acc[t], (unsigned long long)carry[t]);
-        else
+        if (WIDE) {
+            const unsigned long long L = acc[t], H = acch[t];
+            const unsigned long long lo = L + (H << 32);
+            *(ulonglong2*)(mesh + 2 * cell) = make_ulonglong2(lo, (H >> 32) + (lo < L ? 1ull : 0ull));
+        } else {
             mesh[cell] = acc[t];
+        }
     }
 }
 
-// Far particles of every column: workgroup b adds segment b of `far` with global atomics.
+// Exclusive scan of the per-column far counts (one workgroup): offs[b] = sum of farcnt[0..b),
+// offs[nseg] = total.  Sequential per thread over a chunk, then a scan of the thread sums.
+__global__ __launch_bounds__(kThreads) void k_scan_counts(const unsigned int* __restrict__ farcnt, int nseg,
+                                                          unsigned int* __restrict__ offs) {
+    __shared__ unsigned int part[kThreads];
+    const int chunk = (nseg + kThreads - 1) / kThreads;
+    const int b0 = threadIdx.x * chunk, b1 = min(nseg, b0 + chunk);
+    unsigned int s = 0;
+    for (int b = b0; b < b1; ++b) s += farcnt[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int run = 0;
+        for (int t = 0; t < kThreads; ++t) {
+            const unsigned int v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        offs[nseg] = run;
+    }
+    __syncthreads();
+    unsigned int run = part[threadIdx.x];
+    for (int b = b0; b < b1; ++b) {
+        offs[b] = run;
+        run += farcnt[b];
+    }
+}
+
+// Far particles of every column, added with global atomics; threads stride over all of them
+// (segment of far particle t found by binary search in offs), so a column holding most of the far
+// particles (the collapsing core) does not serialise on one workgroup.
 template <class Src, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned long long* __restrict__ mesh,
                                                               const int* __restrict__ far,
-                                                              const unsigned int* __restrict__ farcnt, int seg, int N) {
-    const unsigned int m = farcnt[blockIdx.x];
-    const int* mine = far + (size_t)blockIdx.x * seg;
-    for (unsigned int t = threadIdx.x; t < m; t += kThreads) {
+                                                              const unsigned int* __restrict__ offs, int nseg, int seg,
+                                                              int N) {
+    const unsigned int total = offs[nseg];
+    for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < total; t += gridDim.x * kThreads) {
+        int lo_b = 0, hi_b = nseg - 1;  // last segment b with offs[b] <= t
+        while (lo_b < hi_b) {
+            const int mid = (lo_b + hi_b + 1) >> 1;
+            if (offs[mid] <= t) lo_b = mid;
+            else hi_b = mid - 1;
+        }
+        const long long i = far[(size_t)lo_b * seg + (t - offs[lo_b])];
         int lo[3], hi[3];
-        unsigned long long q[8];
-        src.stencil(src.load(mine[t]), lo, hi, q);
+        typename Src::W w0[3], w1[3];
+        src.axes(src.load(i), lo, hi, w0, w1);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            if (q[c] == 0ull) continue;
+            const unsigned long long v = src.quant(((c >> 2 ? w1[0] : w0[0]) * ((c >> 1) & 1 ? w1[1] : w0[1])) *
+                                                   (c & 1 ? w1[2] : w0[2]));
+            if (v == 0ull) continue;
             const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-            fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, q[c]);
+            fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, v);
         }
     }
 }
@@ -184,7 +237,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned 
 // Column shape per mesh size, cell width and LDS budget: the widest of 8x8, 4x8, 4x4, 2x4
 // columns whose LDS accumulator fits in BUDGET bytes (gfx950 has 160 KB of LDS per CU).
 template <int N, bool WIDE, int BUDGET = 131072> struct ColShape {
-    static constexpr int CELLB = WIDE ? 12 : 8;
+    static constexpr int CELLB = WIDE ? 16 : 8;
     static constexpr bool fits(int bx, int by) { return bx * by * N * CELLB <= BUDGET; }
     static constexpr int BX = fits(8, 8) ? 8 : fits(4, 8) ? 4 : fits(4, 4) ? 4 : 2;
     static constexpr int BY = fits(8, 8) ? 8 : fits(4, 8) ? 8 : fits(4, 4) ? 4 : 4;

@@ -20,20 +20,89 @@
 
 namespace pm3d {
 
-// Pencils per workgroup for each pass: every LDS ping-pong buffer holds about 16 KB, and the count
+// Pencils per workgroup for each pass: every LDS ping-pong buffer holds about 32 KB, and the count
 // is a power of two that divides the number of pencils (N is 2^k or 3*2^k).
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
 constexpr int pow2_dividing(int cap, int total) {  // largest power of two <= cap dividing total
     return (cap <= 1) ? 1 : (total % cap == 0 ? cap : pow2_dividing(cap / 2, total));
 }
-template <typename R, int N> struct Plan {
-    static constexpr int EB = 16384 / (2 * (int)sizeof(R));  // complex elements per buffer
+// Tile configuration V of a pass:
+//   bit 0: ping-pong buffers of ~16 KB (0: more resident workgroups) or ~32 KB (1: more pencils
+//          per workgroup).  The buffer size changes only how many pencils a workgroup holds, so
+//          both give the same bits.
+//   bit 1: largest radix 4 (0) or 8 in FP64 / 16 in FP32 (1): fewer LDS round trips per pencil,
+//          but a different (equally accurate) rounding sequence.
+template <typename R, int N, int V> struct Plan {
+    static constexpr int EB = ((V & 1) ? 32768 : 16384) / (2 * (int)sizeof(R));  // complex elements per buffer
+    static constexpr int ML2 = (V & 2) ? (sizeof(R) == 4 ? 4 : 3) : 2;           // log2 of the largest radix
     static constexpr int PZ = pow2_dividing(pow2_floor(cmin(2 * EB / N, N * N)), N * N);  // r2c rows
     static constexpr int PY = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // y columns
     static constexpr int PX = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // x columns
-    static constexpr int PC = pow2_dividing(pow2_floor(cmin(EB / N, N * N)), N * N);      // c2r rows
+    static constexpr int PC = pow2_dividing(pow2_floor(cmin(((V & 1) ? EB / 2 : EB) / N, N * N)), N * N);  // c2r rows
 };
+
+// In-register DFT of size R in {2, 3, 4, 8, 16} (forward: exp(-2 pi i km/R); INV: conjugate).
+// Sizes 8 and 16 split into even/odd halves (decimation in time) with the constant twiddles
+// exp(-/+ 2 pi i k/R).
+template <typename R, bool INV>
+__device__ __forceinline__ cplx<R> rot_mi(cplx<R> a) {  // multiply by -i (forward) or +i (inverse)
+    return INV ? cplx<R>{-a.im, a.re} : cplx<R>{a.im, -a.re};
+}
+template <typename R, bool INV>
+__device__ __forceinline__ cplx<R> twc(cplx<R> a, double c, double sn) {  // a * (c -/+ i sn)
+    const R cc = (R)c, ss = (R)(INV ? -sn : sn);
+    return cplx<R>{a.re * cc + a.im * ss, a.im * cc - a.re * ss};
+}
+template <typename R, int RADIX, bool INV>
+__device__ __forceinline__ void dft_reg(cplx<R>* v) {
+    if constexpr (RADIX == 2) {
+        const cplx<R> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (RADIX == 3) {
+        const R h = (R)0.86602540378443864676;  // sin(2 pi / 3)
+        const cplx<R> t1 = cadd(v[1], v[2]);
+        const cplx<R> t2 = cplx<R>{v[0].re - (R)0.5 * t1.re, v[0].im - (R)0.5 * t1.im};
+        const cplx<R> d = csub(v[1], v[2]);
+        const cplx<R> t3 = INV ? cplx<R>{-h * d.im, h * d.re} : cplx<R>{h * d.im, -h * d.re};
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(t2, t3);
+        v[2] = csub(t2, t3);
+    } else if constexpr (RADIX == 4) {
+        const cplx<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]), a2 = cadd(v[1], v[3]);
+        const cplx<R> a3 = rot_mi<R, INV>(csub(v[1], v[3]));
+        v[0] = cadd(a0, a2);
+        v[1] = cadd(a1, a3);
+        v[2] = csub(a0, a2);
+        v[3] = csub(a1, a3);
+    } else {
+        constexpr int H = RADIX / 2;
+        cplx<R> e[H], o[H];
+#pragma unroll
+        for (int m = 0; m < H; ++m) {
+            e[m] = v[2 * m];
+            o[m] = v[2 * m + 1];
+        }
+        dft_reg<R, H, INV>(e);
+        dft_reg<R, H, INV>(o);
+        // cos/sin(2 pi k / 16), k = 0..7 (k/16 = 2k/32 for RADIX 8 uses even entries)
+        constexpr double C16[8] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                                   0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613};
+        constexpr double S16[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                                   1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173};
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            constexpr int STEP = 16 / RADIX;
+            cplx<R> t;
+            if (k == 0) t = o[0];
+            else if (k * STEP == 4) t = rot_mi<R, INV>(o[k]);
+            else t = twc<R, INV>(o[k], C16[k * STEP], S16[k * STEP]);
+            v[k] = cadd(e[k], t);
+            v[k + H] = csub(e[k], t);
+        }
+    }
+}
 
 // Stockham pass of radix RADIX on P pencils (see lds_fft).
 template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
@@ -54,31 +123,10 @@ __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, 
                 v[r] = cmul(v[r], w);
             }
         }
-        cplx<R> y[RADIX];
-        if constexpr (RADIX == 4) {
-            const cplx<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]), a2 = cadd(v[1], v[3]);
-            cplx<R> a3 = csub(v[1], v[3]);
-            a3 = INV ? cplx<R>{-a3.im, a3.re} : cplx<R>{a3.im, -a3.re};
-            y[0] = cadd(a0, a2);
-            y[1] = cadd(a1, a3);
-            y[2] = csub(a0, a2);
-            y[3] = csub(a1, a3);
-        } else if constexpr (RADIX == 2) {
-            y[0] = cadd(v[0], v[1]);
-            y[1] = csub(v[0], v[1]);
-        } else {  // radix 3: X1,2 = v0 - (v1+v2)/2 -/+ i sin(2pi/3) (v1-v2) (forward)
-            const R h = (R)0.86602540378443864676;  // sin(2 pi / 3)
-            const cplx<R> t1 = cadd(v[1], v[2]);
-            const cplx<R> t2 = cplx<R>{v[0].re - (R)0.5 * t1.re, v[0].im - (R)0.5 * t1.im};
-            const cplx<R> d = csub(v[1], v[2]);
-            const cplx<R> t3 = INV ? cplx<R>{-h * d.im, h * d.re} : cplx<R>{h * d.im, -h * d.re};  // -/+ i h d
-            y[0] = cadd(v[0], t1);
-            y[1] = cadd(t2, t3);
-            y[2] = csub(t2, t3);
-        }
+        dft_reg<R, RADIX, INV>(v);
         const int dd = (j / Ns) * Ns * RADIX + k;
 #pragma unroll
-        for (int r = 0; r < RADIX; ++r) dst[(dd + r * Ns) * PP + c] = y[r];
+        for (int r = 0; r < RADIX; ++r) dst[(dd + r * Ns) * PP + c] = v[r];
     }
     __syncthreads();
 }
@@ -86,34 +134,21 @@ __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, 
 constexpr int count_factor(int n, int f) { return (n % f == 0) ? 1 + count_factor(n / f, f) : 0; }
 
 // In-LDS Stockham FFT of P pencils of NC points (NC = 2^a 3^b, unnormalised; forward =
-// exp(-2 pi i jk/NC)).  Radix-4 passes, then one radix-2 pass if a is odd, then radix-3 passes.
-// Input in `a`, result in the returned buffer.  Caller synchronises before the call; the call ends
-// with a barrier.  tw[k] = exp(-2 pi i k / (NC*TWS)).
-template <typename R, int NC, int P, bool INV, int TWS>
+// exp(-2 pi i jk/NC)).  Power-of-two passes of radix 2^ML2 (then one smaller power of two for the
+// rest), then radix-3 passes; each pass is one LDS round trip.  Input in `a`, result in the
+// returned buffer.  Caller synchronises before the call; the call ends with a barrier.
+// tw[k] = exp(-2 pi i k / (NC*TWS)).
+template <typename R, int NC, int P, bool INV, int TWS, int ML2, int NS = 1, int REM2 = count_factor(NC, 2),
+          int REM3 = count_factor(NC, 3)>
 __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R>* __restrict__ tw) {
-    constexpr int P2 = count_factor(NC, 2), P3 = count_factor(NC, 3);
-    static_assert((1 << P2) * (P3 == 0 ? 1 : (P3 == 1 ? 3 : (P3 == 2 ? 9 : 27))) == NC, "NC must be 2^a 3^b, b <= 3");
-    cplx<R>* src = a;
-    cplx<R>* dst = b;
-    int Ns = 1;
-#pragma unroll
-    for (int q = 0; q < P2 / 2; ++q) {
-        stockham_pass<R, NC, P, INV, TWS, 4>(src, dst, Ns, tw);
-        Ns *= 4;
-        cplx<R>* tmp = src; src = dst; dst = tmp;
+    if constexpr (REM2 == 0 && REM3 == 0) {
+        return a;
+    } else {
+        constexpr int L2 = REM2 >= ML2 ? ML2 : REM2;
+        constexpr int RADIX = REM2 > 0 ? (1 << L2) : 3;
+        stockham_pass<R, NC, P, INV, TWS, RADIX>(a, b, NS, tw);
+        return lds_fft<R, NC, P, INV, TWS, ML2, NS * RADIX, (REM2 > 0 ? REM2 - L2 : 0), (REM2 > 0 ? REM3 : REM3 - 1)>(b, a, tw);
     }
-    if (P2 & 1) {
-        stockham_pass<R, NC, P, INV, TWS, 2>(src, dst, Ns, tw);
-        Ns *= 2;
-        cplx<R>* tmp = src; src = dst; dst = tmp;
-    }
-#pragma unroll
-    for (int q = 0; q < P3; ++q) {
-        stockham_pass<R, NC, P, INV, TWS, 3>(src, dst, Ns, tw);
-        Ns *= 3;
-        cplx<R>* tmp = src; src = dst; dst = tmp;
-    }
-    return src;
 }
 
 // ------------------------------------------------------------------------------------------ z r2c
@@ -122,12 +157,12 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R
 // with each block's maximum rho stored to bmax[blockIdx.x] (reduced by the following y pass, so no
 // two blocks contend for one address).  Block 0 also clears the deposit's far-particle counter for
 // the next deposit.
-template <typename R, int N, bool FIXED, bool WIDE>
+template <typename R, int N, bool FIXED, bool WIDE, int V>
 __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                     cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
                                                     double* __restrict__ bmax, unsigned long long* __restrict__ nfar,
                                                     double qscale, double nmean) {
-    constexpr int NC = N / 2, P = Plan<R, N>::PZ, PP = P + 1;
+    constexpr int NC = N / 2, P = Plan<R, N, V>::PZ, PP = P + 1;
     __shared__ cplx<R> A[NC * PP], B[NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
     double lmax = 0.0;
@@ -155,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
         }
     }
     __syncthreads();
-    const cplx<R>* Z = lds_fft<R, NC, P, false, 2>(A, B, tw);
+    const cplx<R>* Z = lds_fft<R, NC, P, false, 2, Plan<R, N, V>::ML2>(A, B, tw);
     for (int t = threadIdx.x; t < P * NC; t += kThreads) {
         const int c = t / NC, k = t - c * NC;
         const cplx<R> zk = Z[k * PP + c];
@@ -173,12 +208,12 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
 // In-place complex FFT along y on `ncomp` = gridDim.y half spectra (component strides cs, cn).
 // If nbmax > 0, the last block also reduces the r2c pass's per-block maxima bmax[0..nbmax) into
 // *slot (the deposit's maximum rho, as the bits of a non-negative double).
-template <typename R, int N, bool INV>
+template <typename R, int N, bool INV, int V>
 __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
                                                     size_t cs, size_t cn, const cplx<R>* __restrict__ tw,
                                                     const double* __restrict__ bmax, int nbmax,
                                                     unsigned long long* __restrict__ slot) {
-    constexpr int P = Plan<R, N>::PY, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
+    constexpr int P = Plan<R, N, V>::PY, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
     __shared__ cplx<R> A[N * PP], B[N * PP];
     if (nbmax > 0 && blockIdx.x == gridDim.x - 1 && blockIdx.y == 0) {
         double m = 0.0;
@@ -197,7 +232,7 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
             A[j * PP + c] = base[(size_t)j * NH + c];
         }
         __syncthreads();
-        const cplx<R>* res = lds_fft<R, N, P, INV, 1>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, INV, 1, Plan<R, N, V>::ML2>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;
             base[(size_t)j * NH + c] = res[j * PP + c];
@@ -209,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
             A[j * PP + c] = base[(size_t)c * N + j];
         }
         __syncthreads();
-        const cplx<R>* res = lds_fft<R, N, P, INV, 1>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, INV, 1, Plan<R, N, V>::ML2>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int j = t % N, c = t / N;
             base[(size_t)c * N + j] = res[j * PP + c];
@@ -230,7 +265,7 @@ __device__ __forceinline__ double khalf(int i, double kval) { return 6.283185307
 // (pm3d.py:105-109; numpy's complex/real division is a multiply by the reciprocal), then for
 // c = x,y,z: -1j*k_c*phik = (k_c*Im, -k_c*Re) (pm3d.py:112) and the inverse FFT along x, written to
 // the three gradient spectra.
-template <typename R, int N, int MODE>
+template <typename R, int N, int MODE, int V>
 __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
                                                      cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
                                                      size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
@@ -242,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
         sm2 = tab[k].sm2;
         smooth_on = sm2 > 0.0 ? 1 : 0;
     }
-    constexpr int P = Plan<R, N>::PX, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
+    constexpr int P = Plan<R, N, V>::PX, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
     constexpr int EPT = (N * P + kThreads - 1) / kThreads;
     __shared__ cplx<R> A[N * PP], B[N * PP];
     const int b = blockIdx.x;
@@ -267,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
     if constexpr (MODE == 3) {
         // Zel'dovich displacement: psi_c = 1j*k_c*dk/k2 with dk(0) = 0, evaluated as numpy does
         // ((0 + 1j*k_c) * dk, then complex/real as a multiply by 1/k2), then inverse x FFTs
-        const cplx<R>* res = lds_fft<R, N, P, false, 1>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, false, 1, Plan<R, N, V>::ML2>(A, B, tw);
         cplx<R> dkr[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
@@ -295,7 +330,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
                 }
             }
             __syncthreads();
-            const cplx<R>* r2 = lds_fft<R, N, P, true, 1>(A, B, tw);
+            const cplx<R>* r2 = lds_fft<R, N, P, true, 1, Plan<R, N, V>::ML2>(A, B, tw);
             cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
             for (int t = threadIdx.x; t < N * P; t += kThreads) {
                 const int c = t % P, j = t / P;
@@ -305,14 +340,14 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
         }
         return;
     } else if constexpr (MODE < 2) {
-        const cplx<R>* res = lds_fft<R, N, P, MODE == 1, 1>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, MODE == 1, 1, Plan<R, N, V>::ML2>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;
             src[off(j, c)] = res[j * PP + c];
         }
         return;
     } else {
-        const cplx<R>* res = lds_fft<R, N, P, false, 1>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, false, 1, Plan<R, N, V>::ML2>(A, B, tw);
         cplx<R> ph[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
@@ -350,7 +385,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
                 }
             }
             __syncthreads();
-            const cplx<R>* r2 = lds_fft<R, N, P, true, 1>(A, B, tw);
+            const cplx<R>* r2 = lds_fft<R, N, P, true, 1, Plan<R, N, V>::ML2>(A, B, tw);
             cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
             for (int t = threadIdx.x; t < N * P; t += kThreads) {
                 const int c = t % P, j = t / P;
@@ -365,11 +400,11 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
 // Half spectra along kz -> real rows, NCOMP components written interleaved out[cell*NCOMP + comp],
 // scaled by `scale` (1/N^3, numpy's backward normalisation).  numpy's c2r uses only the real parts
 // of the kz = 0 and kz = N/2 bins (numpy/fft/_pocketfft.py irfft), and so does this kernel.
-template <typename R, int N, int NCOMP>
+template <typename R, int N, int NCOMP, int V>
 __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ gspec, const cplx<R>* __restrict__ gnyq,
                                                     size_t gcs, size_t gcn, R* __restrict__ out,
                                                     const cplx<R>* __restrict__ tw, R scale) {
-    constexpr int NC = N / 2, P = Plan<R, N>::PC, PP = P + 1;
+    constexpr int NC = N / 2, P = Plan<R, N, V>::PC, PP = P + 1;
     __shared__ cplx<R> A[NC * PP], B[NC * PP];
     __shared__ R O[P * N * NCOMP];
     const size_t row0 = (size_t)blockIdx.x * P;
@@ -388,7 +423,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ 
             A[k * PP + c] = cplx<R>{er - oi, ei + orr};
         }
         __syncthreads();
-        const cplx<R>* z = lds_fft<R, NC, P, true, 2>(A, B, tw);
+        const cplx<R>* z = lds_fft<R, NC, P, true, 2, Plan<R, N, V>::ML2>(A, B, tw);
         for (int t = threadIdx.x; t < P * NC; t += kThreads) {
             const int c = t / NC, m = t - c * NC;
             const cplx<R> v = z[m * PP + c];

@@ -1,5 +1,6 @@
-// Particle-side kernels of the pm3d step in FP64 (gfx950): CIC key/fraction, int64 fixed-point
-// deposit, CIC gather fused with the kick, kick+drift, and the Lagrangian-lattice Jacobian.
+// Particle-side kernels of the pm3d step in FP64 (gfx950): CIC key/fraction, the reference-shaped
+// fixed-point atomic deposit, CIC gather, kick+drift, and the Lagrangian-lattice Jacobian.  (The
+// column deposit of the step is in deposit.hpp.)
 //
 // Particle arrays are addressed as x[d*sd + i*si]: SoA device state uses (sd, si) = (n, 1), host
 // (n,3) AoS arrays staged for the reference-shaped entry points use (1, 3).
@@ -38,22 +39,6 @@ __device__ __forceinline__ void cic_weights(const double f[3], double w[8]) {
     for (int c = 0; c < 8; ++c) w[c] = (wx[c >> 2] * wy[(c >> 1) & 1]) * wz[c & 1];
 }
 
-// Fixed-point quantisation of one particle's 8 weights with exact unit mass: every corner is
-// rounded to nearest at 2^-FB except the largest-weight corner, which takes the remainder so that
-// the 8 integers sum to exactly 2^FB.  All values stay >= 0 (the largest weight is >= 1/8).
-// FB <= 60, so every value fits in 63 bits.
-__device__ __forceinline__ void quantise8(const double f[3], const double w[8], double qs, long long one,
-                                          long long q[8]) {
-    const int cmax = ((f[0] > 0.5) << 2) | ((f[1] > 0.5) << 1) | (f[2] > 0.5);
-    long long sum = 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        q[c] = (c == cmax) ? 0 : (long long)u64_of_integral(rint(w[c] * qs));
-        sum += q[c];
-    }
-    q[cmax] = one - sum;
-}
-
 // ------------------------------------------------------------------------------- deposit (atomic)
 // Direct scatter: each particle adds its 8 fixed-point weights with 64-bit integer atomics.
 // Integer addition commutes, so the mesh is bitwise identical whatever the launch order.
@@ -69,123 +54,14 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
 #pragma unroll
     for (int d = 0; d < 3; ++d) cic_axis_fast(X(d, i), dx, N, lo[d], hi[d], f[d]);
     cic_weights(f, w);
-    quantise8(f, w, qs, one, q);
+    // the grid of the mesh: 2^-52 on 128-bit cells, 2^-31 on 64-bit cells (the mixed mode's)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) q[c] = WIDE ? (long long)q52(w[c]) : (long long)rint(w[c] * 2147483648.0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         if (q[c] == 0) continue;
         const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
         fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
-    }
-}
-
-// ------------------------------------------------------------------------------ deposit (tiled)
-// Owner-computes deposit.  The mesh is cut into bricks of BX*BY*BZ cells; the workgroup owning a
-// brick reads every particle whose Lagrangian lattice site lies within HALO sites of the brick
-// (Np == N: site s sits at the centre of cell s), accumulates the corners that land inside its
-// brick in LDS with 64-bit integer atomics, and writes the brick once with plain stores.
-// A particle displaced by HALO cells or more from its site along any axis ("far") is skipped here and
-// appended, by the workgroup owning its site, to `far` for k_deposit_far, which adds it afterwards
-// with global atomics.  The result is bitwise identical to k_deposit_atomic.
-template <int BX, int BY, int BZ, int HALO, bool WIDE, int NT>
-__global__ __launch_bounds__(NT) void k_deposit_brick(const double* __restrict__ x, long long n, int N, double dx,
-                                                      unsigned long long* __restrict__ mesh, double qs, long long one,
-                                                      int* __restrict__ far, unsigned int* __restrict__ nfar) {
-    constexpr int NCELL = BX * BY * BZ;
-    constexpr int EX = BX + 2 * HALO, EY = BY + 2 * HALO, EZ = BZ + 2 * HALO;
-    __shared__ unsigned long long acc[NCELL];
-    __shared__ unsigned int carry[WIDE ? NCELL : 1];
-    for (int t = threadIdx.x; t < NCELL; t += NT) {
-        acc[t] = 0ull;
-        if (WIDE) carry[t] = 0u;
-    }
-    __syncthreads();
-    const int nbz = N / BZ, nby = N / BY;
-    const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
-    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
-    const double* x1 = x + n;
-    const double* x2 = x + 2 * n;
-    for (int t = threadIdx.x; t < EX * EY * EZ; t += NT) {
-        const int ez = t % EZ, ey = (t / EZ) % EY, ex = t / (EZ * EY);
-        const int sx = wrapN(x0 + ex - HALO, N), sy = wrapN(y0 + ey - HALO, N), sz = wrapN(z0 + ez - HALO, N);
-        const long long i = ((long long)sx * N + sy) * N + sz;
-        const double X[3] = {x[i], x1[i], x2[i]};
-        int lo[3], hi[3];
-        double f[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) cic_axis_fast(X[d], dx, N, lo[d], hi[d], f[d]);
-        // displacement of the base cell from the site, minimum image, per axis
-        const int site[3] = {sx, sy, sz};
-        bool is_far = false;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            int dd = lo[d] - site[d];
-            if (dd > N / 2) dd -= N;
-            if (dd < -N / 2) dd += N;
-            // corners lo, lo+1 must stay within [site-HALO, site+HALO]
-            is_far |= (dd < -HALO) || (dd > HALO - 1);
-        }
-        const bool own_site = ex >= HALO && ex < HALO + BX && ey >= HALO && ey < HALO + BY && ez >= HALO && ez < HALO + BZ;
-        // append owned far particles: one atomic per wave, slots by prefix popcount
-        const bool app = is_far && own_site;
-        const unsigned long long bal = __ballot(app);
-        if (bal) {
-            const int lane = threadIdx.x & 63;
-            const int leader = __ffsll((long long)bal) - 1;
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(nfar, (unsigned int)__popcll(bal));
-            base = __shfl(base, leader, 64);
-            if (app) far[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)i;
-        }
-        if (is_far) continue;
-        double w[8];
-        long long q[8];
-        cic_weights(f, w);
-        quantise8(f, w, qs, one, q);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int cx = ((c >> 2) ? hi[0] : lo[0]) - x0, cy = (((c >> 1) & 1) ? hi[1] : lo[1]) - y0,
-                      cz = ((c & 1) ? hi[2] : lo[2]) - z0;
-            // wrap relative coordinates into the brick frame (brick may touch the periodic seam)
-            const int rx = wrapN(cx, N), ry = wrapN(cy, N), rz = wrapN(cz, N);
-            if (rx < BX && ry < BY && rz < BZ && q[c] != 0) {
-                const int cell = (rx * BY + ry) * BZ + rz;
-                const unsigned long long old = atomicAdd(&acc[cell], (unsigned long long)q[c]);
-                if (WIDE && old + (unsigned long long)q[c] < old) atomicAdd(&carry[cell], 1u);
-            }
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NCELL; t += NT) {
-        const int rz = t % BZ, ry = (t / BZ) % BY, rx = t / (BZ * BY);
-        const size_t cell = ((size_t)(x0 + rx) * N + (y0 + ry)) * N + (z0 + rz);
-        if (WIDE)
-            *(ulonglong2*)(mesh + 2 * cell) = make_ulonglong2(acc[t], (unsigned long long)carry[t]);
-        else
-            mesh[cell] = acc[t];
-    }
-}
-
-template <bool WIDE>
-__global__ __launch_bounds__(kThreads) void k_deposit_far(const double* __restrict__ x, long long n, int N, double dx,
-                                                          unsigned long long* __restrict__ mesh, double qs,
-                                                          long long one, const int* __restrict__ far,
-                                                          const unsigned int* __restrict__ nfar) {
-    const unsigned int m = *nfar;
-    for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < m; t += gridDim.x * kThreads) {
-        const long long i = far[t];
-        int lo[3], hi[3];
-        double f[3], w[8];
-        long long q[8];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) cic_axis_fast(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
-        cic_weights(f, w);
-        quantise8(f, w, qs, one, q);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            if (q[c] == 0) continue;
-            const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-            fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, (unsigned long long)q[c]);
-        }
     }
 }
 

@@ -3,14 +3,17 @@
 //     x = (cell + 1/2 + off) * dx,   cell int32 in [0, N),   off float in [0, 1)
 // so the deposit and the gather need no division and the weights are (1-off, off).  Momenta and
 // accelerations are FP32; the scalars of the step arrive from the host in FP64 and are folded into
-// FP32 factors once per launch.  The deposit mesh is one uint64 per cell at 2^-32: the whole box
-// holds n * 2^32 < 2^63 units for n < 2^31, so it cannot overflow.
+// FP32 factors once per launch.  The deposit mesh is one uint64 per cell at 2^-31: the whole box
+// holds at most n * 2^31 < 2^63 units for n < 2^32, so it cannot overflow.
 #pragma once
 #include "common.hpp"
 
 namespace pm3d {
 
-constexpr int kNarrowFB = 32;
+constexpr int kNarrowFB = 31;
+
+// Narrow fixed point: a weight w in [0, 1] as round(w * 2^31), one FP32 -> u32 conversion.
+__device__ __forceinline__ unsigned int q31(float w) { return (unsigned int)rintf(w * 2147483648.0f); }
 
 __device__ __forceinline__ void mx_weights(const float f[3], float w[8]) {
     const float wx[2] = {1.0f - f[0], f[0]}, wy[2] = {1.0f - f[1], f[1]}, wz[2] = {1.0f - f[2], f[2]};
@@ -18,16 +21,11 @@ __device__ __forceinline__ void mx_weights(const float f[3], float w[8]) {
     for (int c = 0; c < 8; ++c) w[c] = (wx[c >> 2] * wy[(c >> 1) & 1]) * wz[c & 1];
 }
 
-// 8 fixed-point weights summing to exactly 2^32 (largest corner takes the remainder).
+// The 8 corner weights, each rounded to nearest at 2^-31 independently.
 __device__ __forceinline__ void mx_quantise8(const float f[3], const float w[8], unsigned long long q[8]) {
-    const int cmax = ((f[0] > 0.5f) << 2) | ((f[1] > 0.5f) << 1) | (f[2] > 0.5f);
-    unsigned long long sum = 0;
+    (void)f;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        q[c] = (c == cmax) ? 0ull : u64_of_integral((double)rintf(w[c] * 4294967296.0f));
-        sum += q[c];
-    }
-    q[cmax] = (1ull << kNarrowFB) - sum;
+    for (int c = 0; c < 8; ++c) q[c] = q31(w[c]);
 }
 
 struct MxPos {
@@ -42,63 +40,6 @@ struct MxPos {
         }
     }
 };
-
-// Owner-computes brick deposit (see k_deposit_brick in particles.hpp for the scheme).
-template <int BX, int BY, int BZ, int HALO, int NT>
-__global__ __launch_bounds__(NT) void k_deposit_brick_mx(MxPos P, int N, unsigned long long* __restrict__ mesh,
-                                                         int* __restrict__ far, unsigned int* __restrict__ nfar) {
-    constexpr int NCELL = BX * BY * BZ;
-    constexpr int EX = BX + 2 * HALO, EY = BY + 2 * HALO, EZ = BZ + 2 * HALO;
-    __shared__ unsigned long long acc[NCELL];
-    for (int t = threadIdx.x; t < NCELL; t += NT) acc[t] = 0ull;
-    __syncthreads();
-    const int nbz = N / BZ, nby = N / BY;
-    const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
-    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
-    for (int t = threadIdx.x; t < EX * EY * EZ; t += NT) {
-        const int ez = t % EZ, ey = (t / EZ) % EY, ex = t / (EZ * EY);
-        const int site[3] = {wrapN(x0 + ex - HALO, N), wrapN(y0 + ey - HALO, N), wrapN(z0 + ez - HALO, N)};
-        const long long i = ((long long)site[0] * N + site[1]) * N + site[2];
-        int lo[3];
-        float f[3];
-        P.load(i, lo, f);
-        bool is_far = false;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            int dd = lo[d] - site[d];
-            if (dd > N / 2) dd -= N;
-            if (dd < -N / 2) dd += N;
-            is_far |= (dd < -HALO) || (dd > HALO - 1);
-        }
-        const bool own_site = ex >= HALO && ex < HALO + BX && ey >= HALO && ey < HALO + BY && ez >= HALO && ez < HALO + BZ;
-        const bool app = is_far && own_site;
-        const unsigned long long bal = __ballot(app);
-        if (bal) {
-            const int lane = threadIdx.x & 63;
-            const int leader = __ffsll((long long)bal) - 1;
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(nfar, (unsigned int)__popcll(bal));
-            base = __shfl(base, leader, 64);
-            if (app) far[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)i;
-        }
-        if (is_far) continue;
-        float w[8];
-        unsigned long long q[8];
-        mx_weights(f, w);
-        mx_quantise8(f, w, q);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int cx = (c >> 2) ? lo[0] + 1 : lo[0], cy = ((c >> 1) & 1) ? lo[1] + 1 : lo[1], cz = (c & 1) ? lo[2] + 1 : lo[2];
-            const int rx = wrapN(cx - x0, N), ry = wrapN(cy - y0, N), rz = wrapN(cz - z0, N);
-            if (rx < BX && ry < BY && rz < BZ && q[c] != 0) atomicAdd(&acc[(rx * BY + ry) * BZ + rz], q[c]);
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NCELL; t += NT) {
-        const int rz = t % BZ, ry = (t / BZ) % BY, rx = t / (BZ * BY);
-        mesh[((size_t)(x0 + rx) * N + (y0 + ry)) * N + (z0 + rz)] = acc[t];
-    }
-}
 
 __device__ __forceinline__ void mx_scatter(const int lo[3], const float f[3], int N, unsigned long long* mesh) {
     float w[8];

@@ -13,6 +13,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -99,6 +103,7 @@ struct Handle {
     double t_ms[KC_COUNT] = {};
     std::string err;
     size_t bytes = 0;
+    int fftv[5] = {};  // FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z)
 };
 
 #define HCHK(h, call)                                                     \
@@ -159,77 +164,132 @@ int check_launch(Handle* h, const char* what) {
 }
 
 // ---------------------------------------------------------------- FFT launches, R and N templated
+// Each pass runs in tile variant h->fftv[pass] (fft.hpp Plan<R, N, V>); every variant computes
+// the same bits, so the choice is made by timing (tune_fft).
+enum FftPass { FP_R2C = 0, FP_YFWD, FP_X, FP_YINV, FP_C2R, FP_COUNT };
+
+#define PM3D_V4(v, fn, ...)                   \
+    switch (v) {                              \
+        case 1: fn<1>(__VA_ARGS__); break;    \
+        case 2: fn<2>(__VA_ARGS__); break;    \
+        case 3: fn<3>(__VA_ARGS__); break;    \
+        default: fn<0>(__VA_ARGS__); break;   \
+    }
+
 template <typename R, int N>
 struct Fft {
     static constexpr size_t NH = N / 2, specN = (size_t)N * N * NH, tot = specN + (size_t)N * N;
 
-    // r2c z + forward y on h->spec.  fixed: input is the deposit mesh.
-    static void forward(Handle* h, const void* in, bool fixed) {
+    template <int V>
+    static unsigned r2c_grid() { return (unsigned)((size_t)N * N / Plan<R, N, V>::PZ); }
+
+    template <int V>
+    static void r2c(Handle* h, const void* in, bool fixed) {
         cplx<R>* spec = (cplx<R>*)h->spec;
-        cplx<R>* nyq = spec + specN;
         const cplx<R>* tw = (const cplx<R>*)h->tw;
-        const unsigned gz = (unsigned)((size_t)N * N / Plan<R, N>::PZ);
-        {
-            Timed t(h, KC_R2C_Z);
-            if (fixed && h->wide)
-                k_r2c_z<R, N, true, true><<<gz, kThreads, 0, h->st>>>(in, spec, nyq, tw, h->bmax, h->dscal, h->qscale, h->nmean);
-            else if (fixed)
-                k_r2c_z<R, N, true, false><<<gz, kThreads, 0, h->st>>>(in, spec, nyq, tw, h->bmax, h->dscal, h->qscale, h->nmean);
-            else
-                k_r2c_z<R, N, false, false><<<gz, kThreads, 0, h->st>>>(in, spec, nyq, tw, nullptr, nullptr, 0.0, 1.0);
-        }
-        {
-            Timed t(h, KC_Y_FWD);
-            constexpr int P = Plan<R, N>::PY;
-            dim3 g(N * (N / 2 / P) + N / P, 1);
-            k_c2c_y<R, N, false><<<g, kThreads, 0, h->st>>>(spec, nyq, 0, 0, tw, h->bmax, fixed ? (int)gz : 0, qmax_slot(h));
-        }
+        const unsigned gz = r2c_grid<V>();
+        Timed t(h, KC_R2C_Z);
+        if (fixed && h->wide)
+            k_r2c_z<R, N, true, true, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal,
+                                                                    h->qscale, h->nmean);
+        else if (fixed)
+            k_r2c_z<R, N, true, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal,
+                                                                     h->qscale, h->nmean);
+        else
+            k_r2c_z<R, N, false, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, nullptr,
+                                                                      0.0, 1.0);
     }
 
-    // x pass: mode 0 forward, 1 inverse (in place on h->spec), 2 Poisson (h->spec -> h->gspec)
-    static void xpass(Handle* h, int mode, double s, double sm2) {
+    // nbmax: number of per-block maxima the r2c launch left in h->bmax (0 when the input was real)
+    template <int V>
+    static void yfwd(Handle* h, int nbmax) {
+        cplx<R>* spec = (cplx<R>*)h->spec;
+        constexpr int P = Plan<R, N, V>::PY;
+        dim3 g(N * (N / 2 / P) + N / P, 1);
+        Timed t(h, KC_Y_FWD);
+        k_c2c_y<R, N, false, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, 0, 0, (const cplx<R>*)h->tw, h->bmax,
+                                                           nbmax, qmax_slot(h));
+    }
+
+    // x pass: mode 0 forward, 1 inverse (in place on h->spec), 2 Poisson (h->spec -> h->gspec),
+    // 3 Zel'dovich displacement spectra (h->spec -> h->gspec)
+    template <int V>
+    static void x(Handle* h, int mode, double s, double sm2) {
         cplx<R>* spec = (cplx<R>*)h->spec;
         cplx<R>* gs = (cplx<R>*)h->gspec;
         const cplx<R>* tw = (const cplx<R>*)h->tw;
-        constexpr int P = Plan<R, N>::PX;
+        constexpr int P = Plan<R, N, V>::PX;
         const unsigned g = N * (N / 2 / P) + N / P;
         const double kval = 1.0 / ((double)N * h->dx);
         Timed t(h, KC_X_POISSON);
         const DevSched* sch = h->use_sched ? h->sched : nullptr;
         if (mode == 0)
-            k_x_pass<R, N, 0><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0,
-                                                         nullptr, nullptr);
+            k_x_pass<R, N, 0, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0,
+                                                            0, nullptr, nullptr);
         else if (mode == 1)
-            k_x_pass<R, N, 1><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0,
-                                                         nullptr, nullptr);
+            k_x_pass<R, N, 1, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0,
+                                                            0, nullptr, nullptr);
         else if (mode == 2)
-            k_x_pass<R, N, 2><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s, sm2,
-                                                         sm2 > 0.0 ? 1 : 0, sch, h->tab);
+            k_x_pass<R, N, 2, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s,
+                                                            sm2, sm2 > 0.0 ? 1 : 0, sch, h->tab);
         else
-            k_x_pass<R, N, 3><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, 0, 0, 0,
-                                                         nullptr, nullptr);
+            k_x_pass<R, N, 3, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, 0,
+                                                            0, 0, nullptr, nullptr);
+    }
+
+    template <int V>
+    static void yinv(Handle* h, cplx<R>* base, int ncomp) {
+        constexpr int P = Plan<R, N, V>::PY;
+        dim3 g(N * (N / 2 / P) + N / P, ncomp);
+        Timed t(h, KC_Y_INV);
+        k_c2c_y<R, N, true, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (const cplx<R>*)h->tw, nullptr,
+                                                          0, nullptr);
+    }
+
+    template <int V>
+    static void c2r(Handle* h, cplx<R>* base, int ncomp, void* out) {
+        const unsigned g = (unsigned)((size_t)N * N / Plan<R, N, V>::PC);
+        const R scale = (R)(1.0 / ((double)N * N * N));
+        const cplx<R>* tw = (const cplx<R>*)h->tw;
+        Timed t(h, KC_C2R_Z);
+        if (ncomp == 3)
+            k_c2r_z<R, N, 3, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+        else
+            k_c2r_z<R, N, 1, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+    }
+
+    // r2c z + forward y on h->spec.  fixed: input is the deposit mesh.
+    static void forward(Handle* h, const void* in, bool fixed) {
+        PM3D_V4(h->fftv[FP_R2C], r2c, h, in, fixed);
+        const int nb = fixed ? (int)((h->fftv[FP_R2C] & 1) ? r2c_grid<1>() : r2c_grid<0>()) : 0;
+        PM3D_V4(h->fftv[FP_YFWD], yfwd, h, nb);
+    }
+
+    static void xpass(Handle* h, int mode, double s, double sm2) {
+        PM3D_V4(h->fftv[FP_X], x, h, mode, s, sm2);
     }
 
     // inverse y (ncomp spectra from base) + c2r z into out (ncomp interleaved)
     static void inverse(Handle* h, void* base_, int ncomp, void* out) {
         cplx<R>* base = (cplx<R>*)base_;
-        const cplx<R>* tw = (const cplx<R>*)h->tw;
-        {
-            Timed t(h, KC_Y_INV);
-            constexpr int P = Plan<R, N>::PY;
-            dim3 g(N * (N / 2 / P) + N / P, ncomp);
-            k_c2c_y<R, N, true><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, tw, nullptr, 0, nullptr);
-        }
-        {
-            Timed t(h, KC_C2R_Z);
-            const unsigned g = (unsigned)((size_t)N * N / Plan<R, N>::PC);
-            const R scale = (R)(1.0 / ((double)N * N * N));
-            if (ncomp == 3)
-                k_c2r_z<R, N, 3><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
-            else
-                k_c2r_z<R, N, 1><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+        PM3D_V4(h->fftv[FP_YINV], yinv, h, base, ncomp);
+        PM3D_V4(h->fftv[FP_C2R], c2r, h, base, ncomp, out);
+    }
+
+    // Launches pass `pass` in variant V as the step does (fixed-point r2c input, Poisson x pass,
+    // three components inverse).
+    template <int V>
+    static void one(Handle* h, int pass) {
+        switch (pass) {
+            case FP_R2C: r2c<V>(h, h->mesh, true); break;
+            case FP_YFWD: yfwd<V>(h, (int)r2c_grid<V>()); break;
+            case FP_X: x<V>(h, 2, 1.0, 0.0); break;
+            case FP_YINV: yinv<V>(h, (cplx<R>*)h->gspec, 3); break;
+            default: c2r<V>(h, (cplx<R>*)h->gspec, 3, h->grad); break;
         }
     }
+
+    static void launch_variant(Handle* h, int pass, int v) { PM3D_V4(v, one, h, pass); }
 };
 
 template <typename R>
@@ -241,8 +301,10 @@ int fft_call(Handle* h, int op, const void* in, bool fixed, int mode, double s, 
             Fft<R, NN>::forward(h, in, fixed);                \
         else if (op == 1)                                     \
             Fft<R, NN>::xpass(h, mode, s, sm2);               \
-        else                                                  \
+        else if (op == 2)                                     \
             Fft<R, NN>::inverse(h, base, ncomp, out);         \
+        else                                                  \
+            Fft<R, NN>::launch_variant(h, mode, ncomp);       \
         break;
     switch (h->N) {
         PM3D_FFT_CASE(8)
@@ -273,6 +335,76 @@ int fft_inverse(Handle* h, void* base, int ncomp, void* out) {
     return h->mixed ? fft_call<float>(h, 2, nullptr, false, 0, 0, 0, base, ncomp, out)
                     : fft_call<double>(h, 2, nullptr, false, 0, 0, 0, base, ncomp, out);
 }
+int fft_launch_variant(Handle* h, int pass, int v) {
+    return h->mixed ? fft_call<float>(h, 3, nullptr, false, pass, 0, 0, nullptr, v, nullptr)
+                    : fft_call<double>(h, 3, nullptr, false, pass, 0, 0, nullptr, v, nullptr);
+}
+
+// Picks the faster buffer size (Plan bit 0) of each FFT pass by timing both on this handle's
+// buffers (5 launches each after one warm-up, on zeroed buffers); the radix (Plan bit 1) is fixed
+// per precision and pass by kFftRadixBit, so the choice never changes the results.  The choice is
+// cached per (N, precision) for the process.  PM3D_FFT_TILES = one digit 0-3 forces that variant
+// for every pass, five digits one per pass (r2c_z, y fwd, x, y inv, c2r_z).
+// Radix table from MI355X timings (128^3 FP64, 256^3 FP32; us per launch, radix 4 -> 8/16):
+//   FP64: x 25.9 -> 24.9, y inv 24.2 -> 23.7, the others equal within 1%.
+//   FP32: r2c 56.9 -> 51.9, y fwd 45.0 -> 44.2, y inv 122 -> 114, c2r 171 -> 148; x 99 -> 116
+//   (the Poisson x pass keeps its per-thread spectrum in registers, which radix 16 crowds).
+constexpr int kFftRadixBit[2][FP_COUNT] = {{0, 0, 1, 1, 0}, {1, 1, 0, 1, 1}};
+std::mutex g_tune_mu;
+std::map<std::pair<int, bool>, std::array<int, FP_COUNT>> g_tuned;
+
+int tune_fft(Handle* h) {
+    const char* env = std::getenv("PM3D_FFT_TILES");
+    if (env && *env && std::strcmp(env, "auto") != 0) {
+        const size_t len = std::strlen(env);
+        if (len != 1 && len != FP_COUNT) return fail(h, PM3D_EINVAL, "PM3D_FFT_TILES must be auto, one digit or 5 digits");
+        for (int p = 0; p < FP_COUNT; ++p) {
+            const char c = env[len == 1 ? 0 : p];
+            if (c < '0' || c > '3') return fail(h, PM3D_EINVAL, "PM3D_FFT_TILES digits must be 0-3");
+            h->fftv[p] = c - '0';
+        }
+        return 0;
+    }
+    const std::pair<int, bool> key{h->N, h->mixed};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            for (int p = 0; p < FP_COUNT; ++p) h->fftv[p] = it->second[p];
+            return 0;
+        }
+    }
+    hipEvent_t e0, e1;
+    HCHK(h, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return fail(h, PM3D_EHIP, "hipEventCreate failed");
+    }
+    int rc = 0;
+    std::array<int, FP_COUNT> best{};
+    for (int p = 0; p < FP_COUNT && !rc; ++p) {
+        float ms[2] = {0.f, 0.f};
+        const int rb = kFftRadixBit[h->mixed ? 1 : 0][p] << 1;
+        for (int v = 0; v < 2 && !rc; ++v) {
+            if ((rc = fft_launch_variant(h, p, rb | v))) break;
+            if (hipEventRecord(e0, h->st) != hipSuccess) { rc = fail(h, PM3D_EHIP, "hipEventRecord failed"); break; }
+            for (int r = 0; r < 5 && !rc; ++r) rc = fft_launch_variant(h, p, rb | v);
+            if (rc) break;
+            if (hipEventRecord(e1, h->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms[v], e0, e1) != hipSuccess)
+                rc = fail(h, PM3D_EHIP, "FFT tile timing failed");
+            if (!rc) rc = check_launch(h, "FFT tile timing");
+        }
+        best[p] = rb | (ms[1] < ms[0] ? 1 : 0);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    for (int p = 0; p < FP_COUNT; ++p) h->fftv[p] = best[p];
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[key] = best;
+    return 0;
+}
 
 int ensure_sbuf(Handle* h, size_t bytes) {
     if (h->sbuf_bytes >= bytes) return 0;
@@ -294,7 +426,9 @@ void deposit_columns(Handle* h, const Src& src) {
         k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024><<<S::NBLK, 1024, 0, h->st>>>(src, h->mesh, h->far, h->farcnt);
     }
     Timed t(h, KC_DEPOSIT_FAR);
-    k_deposit_far_seg<Src, WIDE><<<S::NBLK, kThreads, 0, h->st>>>(src, h->mesh, h->far, h->farcnt, S::BX * S::BY * N, N);
+    unsigned int* offs = h->farcnt + S::NBLK;
+    k_scan_counts<<<1, kThreads, 0, h->st>>>(h->farcnt, S::NBLK, offs);
+    k_deposit_far_seg<Src, WIDE><<<512, kThreads, 0, h->st>>>(src, h->mesh, h->far, offs, S::NBLK, S::BX * S::BY * N, N);
 }
 
 template <bool WIDE, class Src>
@@ -568,10 +702,10 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         return PM3D_EINVAL;
     }
     const bool mixed = cfg->precision == PM3D_MIXED;
-    // FP64 mode: 128-bit cells at 2^-60.  Mixed mode: 64-bit cells at 2^-32 (cannot overflow).
+    // FP64 mode: 128-bit cells at 2^-52.  Mixed mode: 64-bit cells at 2^-31 (cannot overflow).
     const int fb = mixed ? kNarrowFB : kWideFB;
     if (cfg->frac_bits != 0 && cfg->frac_bits != fb) {
-        g_create_error = "frac_bits is fixed by the precision mode (FP64: 60 on 128-bit cells, mixed: 32); pass 0";
+        g_create_error = "frac_bits is fixed by the precision mode (FP64: 52 on 128-bit cells, mixed: 31); pass 0";
         return PM3D_EINVAL;
     }
     Handle* h = new Handle();
@@ -616,7 +750,7 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         (rc = dalloc_bytes(h, &h->gspec, 3 * specN * 2 * rb)) || (rc = dalloc_bytes(h, &h->grad, 3 * h->M * rb)) ||
         (rc = dalloc_bytes(h, &h->tw, (size_t)N * 2 * rb)) || (rc = dalloc(h, &h->dscal, 8 + kRing)) ||
         (rc = dalloc(h, &h->bmax, h->bmax_len)) || (rc = dalloc(h, &h->far, (size_t)h->n)) ||
-        (rc = dalloc(h, &h->farcnt, (size_t)N * N)))
+        (rc = dalloc(h, &h->farcnt, 2 * (size_t)N * N + 1)))  // counts and their exclusive scan
         return bail(rc);
     // twiddles exp(-2 pi i k/N) from long-double cos/sin
     std::vector<double> tw(2 * N);
@@ -630,7 +764,17 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
     }
     if (hipMemcpy(h->tw, mixed ? (void*)twf.data() : (void*)tw.data(), N * 2 * rb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(h->dscal, 0, (8 + kRing) * sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(mixed ? (void*)h->accf : (void*)h->acc, 0, n3 * rb) != hipSuccess) {
+        hipMemset(mixed ? (void*)h->accf : (void*)h->acc, 0, n3 * rb) != hipSuccess ||
+        hipMemset(h->mesh, 0, (h->wide ? 16 : 8) * (size_t)h->M) != hipSuccess ||
+        hipMemset(h->spec, 0, specN * 2 * rb) != hipSuccess || hipMemset(h->gspec, 0, 3 * specN * 2 * rb) != hipSuccess ||
+        hipMemset(h->farcnt, 0, (2 * (size_t)N * N + 1) * sizeof(unsigned int)) != hipSuccess) {
+        h->err = "initial copies failed";
+        return bail(PM3D_EHIP);
+    }
+    if ((rc = tune_fft(h))) return bail(rc);
+    // the timing launches wrote a deposit maximum into the ring
+    if (hipMemsetAsync(h->dscal, 0, (8 + kRing) * sizeof(unsigned long long), h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess) {
         h->err = "initial copies failed";
         return bail(PM3D_EHIP);
     }
@@ -949,6 +1093,13 @@ int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax) 
 }
 
 const char* pm3d_kernel_names(void) { return kKernelNames; }
+
+int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax) {
+    if (!handle || !variants || nmax < 0) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    for (int p = 0; p < FP_COUNT && p < nmax; ++p) variants[p] = h->fftv[p];
+    return FP_COUNT;
+}
 
 int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic) {
     if (!handle || !ic) return PM3D_EINVAL;

@@ -358,6 +358,12 @@ class PM3D:
     def set_timing(self, on: bool):
         check(self._L.pm3d_set_timing(self._h, 1 if on else 0), self._h)
 
+    def fft_tiles(self) -> dict:
+        """FFT tile variant per pass chosen at creation (0: 16 KB buffers, radix <= 4; 1: 32 KB, radix 8/16)."""
+        v = np.zeros(5, np.int32)
+        check(self._L.pm3d_fft_tiles(self._h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 5), self._h)
+        return dict(zip(("r2c_z", "y_fwd", "x", "y_inv", "c2r_z"), (int(t) for t in v)))
+
     def timings(self) -> dict:
         names = self._L.pm3d_kernel_names().decode().split(",")
         cnt = np.zeros(len(names), np.int64)
