@@ -118,6 +118,19 @@ int pm3d_min_jacobian(void* handle, double* jmin);
 int pm3d_set_timing(void* handle, int enable);
 int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax);
 const char* pm3d_kernel_names(void);  /* comma-separated */
+/* Diagnostics of inner Lagrangian sets on the device (analyse, pm3d.py:132-159).
+ * pm3d_set_analysis_sets uploads nsets index lists (ascending particle indices, set s =
+ * idx[offsets[s]..offsets[s+1])): set 0 is the core whose centre of mass is the centre
+ * ((xs - xs[0] + L/2) % L - L/2 averaged, pm3d.py:137-141), sets 1.. the inner Lagrangian sets
+ * (pm3d.py:144-146).  pm3d_analyse returns centre[3]; moments[(s-1)*9 + 0..8] for s >= 1 = the set
+ * mean of rel = (x - centre + L/2) % L - L/2 and the sums xx,xy,xz,yy,yz,zz of (rel - mean)
+ * outer products (the reference's r.T @ r before / len(r), pm3d.py:147-148); counts[k] = number of
+ * particles with |rel| < radii[k] (pm3d.py:153-155, nradii <= 16); rho_max = maximum of a deposit
+ * of the current state (pm3d.py:158; the last step's density mesh is left untouched). */
+int pm3d_set_analysis_sets(void* handle, const int32_t* idx, const int64_t* offsets, int nsets);
+int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, double* moments, int64_t* counts,
+                 double* rho_max);
+
 /* FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z), two bits: bit 0 = 32 KB rather than
  * 16 KB LDS buffers (more pencils per workgroup; same bits), bit 1 = radix 8 (FP64) / 16 (FP32)
  * rather than 4 (a different rounding sequence).  pm3d_create times both buffer sizes and keeps

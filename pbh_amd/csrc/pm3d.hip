@@ -30,6 +30,7 @@
 #include "particles_mixed.hpp"
 #include "deposit.hpp"
 #include "ic.hpp"
+#include "analyse.hpp"
 
 using namespace pm3d;
 
@@ -104,6 +105,13 @@ struct Handle {
     std::string err;
     size_t bytes = 0;
     int fftv[5] = {};  // FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z)
+    // analysis sets (pm3d_set_analysis_sets): indices, offsets, centre/moment/partial buffers
+    int* an_idx = nullptr;
+    long long* an_off = nullptr;
+    double* an_buf = nullptr;  // [0..3) centre, [3..4) rho max, [4 + 9*s) per set, then partial sums
+    unsigned long long* an_cnt = nullptr;
+    int an_nsets = 0;
+    long long an_off_host1 = 0;  // size of the core set
 };
 
 #define HCHK(h, call)                                                     \
@@ -419,29 +427,29 @@ int ensure_sbuf(Handle* h, size_t bytes) {
 
 // Column deposit (deposit.hpp) for one mesh size.
 template <int N, bool WIDE, class Src>
-void deposit_columns(Handle* h, const Src& src) {
+void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh) {
     using S = ColShape<N, WIDE>;
     {
         Timed t(h, KC_DEPOSIT);
-        k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024><<<S::NBLK, 1024, 0, h->st>>>(src, h->mesh, h->far, h->farcnt);
+        k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024><<<S::NBLK, 1024, 0, h->st>>>(src, mesh, h->far, h->farcnt);
     }
     Timed t(h, KC_DEPOSIT_FAR);
     unsigned int* offs = h->farcnt + S::NBLK;
     k_scan_counts<<<1, kThreads, 0, h->st>>>(h->farcnt, S::NBLK, offs);
-    k_deposit_far_seg<Src, WIDE><<<512, kThreads, 0, h->st>>>(src, h->mesh, h->far, offs, S::NBLK, S::BX * S::BY * N, N);
+    k_deposit_far_seg<Src, WIDE><<<512, kThreads, 0, h->st>>>(src, mesh, h->far, offs, S::NBLK, S::BX * S::BY * N, N);
 }
 
 template <bool WIDE, class Src>
-int deposit_columns_n(Handle* h, const Src& src) {
+int deposit_columns_n(Handle* h, const Src& src, unsigned long long* mesh) {
     switch (h->N) {
-        case 16: deposit_columns<16, WIDE>(h, src); break;
-        case 32: deposit_columns<32, WIDE>(h, src); break;
-        case 48: deposit_columns<48, WIDE>(h, src); break;
-        case 64: deposit_columns<64, WIDE>(h, src); break;
-        case 96: deposit_columns<96, WIDE>(h, src); break;
-        case 128: deposit_columns<128, WIDE>(h, src); break;
-        case 256: deposit_columns<256, WIDE>(h, src); break;
-        case 512: deposit_columns<512, WIDE>(h, src); break;
+        case 16: deposit_columns<16, WIDE>(h, src, mesh); break;
+        case 32: deposit_columns<32, WIDE>(h, src, mesh); break;
+        case 48: deposit_columns<48, WIDE>(h, src, mesh); break;
+        case 64: deposit_columns<64, WIDE>(h, src, mesh); break;
+        case 96: deposit_columns<96, WIDE>(h, src, mesh); break;
+        case 128: deposit_columns<128, WIDE>(h, src, mesh); break;
+        case 256: deposit_columns<256, WIDE>(h, src, mesh); break;
+        case 512: deposit_columns<512, WIDE>(h, src, mesh); break;
         default: return fail(h, PM3D_EUNSUPPORTED, "column deposit needs N in [16,512]");
     }
     return 0;
@@ -450,32 +458,37 @@ int deposit_columns_n(Handle* h, const Src& src) {
 // Deposit of the device state into h->mesh.  The column path (Np == N, N >= 16) writes every cell;
 // the atomic path clears the mesh first.  This deposit's maximum is stored into its ring slot by the
 // y pass of the FFT.
-int deposit_state(Handle* h) {
+// Deposit of the device state into `mesh` (h->mesh, or a scratch mesh of the same size).
+int deposit_into(Handle* h, unsigned long long* mesh) {
     const bool tiled = h->Np == h->N && h->N >= 16;  // a column plus its one-site ring must not wrap onto itself
-    h->serial += 1;
     const int W = h->wide ? 2 : 1;
     if (!tiled) {
         Timed t(h, KC_FILL);
-        k_fill_u64<<<grid_cap(W * h->M), kThreads, 0, h->st>>>(h->mesh, W * h->M, 0ull);
+        k_fill_u64<<<grid_cap(W * h->M), kThreads, 0, h->st>>>(mesh, W * h->M, 0ull);
     }
     int rc = 0;
     if (!h->mixed) {
         if (tiled) {
-            rc = deposit_columns_n<true>(h, SrcF64{h->x, h->n, h->dx, h->qs, h->qone, h->N});
+            rc = deposit_columns_n<true>(h, SrcF64{h->x, h->n, h->dx, h->qs, h->qone, h->N}, mesh);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic<true>
-                <<<grid_for(h->n), kThreads, 0, h->st>>>(Strided{h->x, h->n, 1}, h->n, h->N, h->dx, h->mesh, h->qs, h->qone);
+                <<<grid_for(h->n), kThreads, 0, h->st>>>(Strided{h->x, h->n, 1}, h->n, h->N, h->dx, mesh, h->qs, h->qone);
         }
     } else {
         if (tiled) {
-            rc = deposit_columns_n<false>(h, SrcMx{h->cell, h->off, h->n, h->N});
+            rc = deposit_columns_n<false>(h, SrcMx{h->cell, h->off, h->n, h->N}, mesh);
         } else {
             Timed t(h, KC_DEPOSIT);
-            k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, h->mesh);
+            k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, mesh);
         }
     }
     return rc ? rc : check_launch(h, "deposit");
+}
+
+int deposit_state(Handle* h) {
+    h->serial += 1;
+    return deposit_into(h, h->mesh);
 }
 
 // forces at the current positions: deposit, Poisson, gather into acc_out (SoA 3n, state precision).
@@ -518,7 +531,8 @@ int read_bad(Handle* h, int* bad) {
 
 void free_all(Handle* h) {
     void* bufs[] = {h->x, h->p, h->acc, h->cell, h->off, h->pf, h->accf, h->mesh, h->spec, h->gspec,
-                    h->grad, h->tw, h->dscal, h->far, h->farcnt, h->bmax, h->sbuf, h->sched, h->tab, h->jhist};
+                    h->grad, h->tw, h->dscal, h->far, h->farcnt, h->bmax, h->sbuf, h->sched, h->tab, h->jhist,
+                    h->an_idx, h->an_off, h->an_buf, h->an_cnt};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& u : h->ev_used) {
@@ -673,6 +687,26 @@ void batch_clear_graph(Batch* b) {
     if (b->graph) hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+
+template <class Pos>
+void launch_analyse(Handle* h, const Pos& pos, const AnRadii& radii) {
+    const double L = h->L, hL = h->L / 2;
+    const int ns = h->an_nsets - 1;
+    double* centre = h->an_buf;
+    double* out = h->an_buf + 4;
+    double* part = out + 9 * ns;
+    double* part2 = part + (size_t)kAnBlocks * 3 * ns;
+    Timed t(h, KC_OTHER);
+    k_an_centre<<<1, 64, 0, h->st>>>(pos, h->an_idx, (int)(h->an_off_host1), L, hL, centre);
+    if (ns > 0) {
+        const dim3 g(kAnBlocks, ns);
+        k_an_sum<<<g, kThreads, 0, h->st>>>(pos, h->an_idx, h->an_off + 1, centre, L, hL, part);
+        k_an_moments<<<g, kThreads, 0, h->st>>>(pos, h->an_idx, h->an_off + 1, centre, L, hL, part, part2);
+        k_an_finish<<<ns, 64, 0, h->st>>>(h->an_off + 1, part, part2, out);
+    }
+    k_fill_u64<<<1, kThreads, 0, h->st>>>(h->an_cnt, kAnMaxRadii, 0ull);
+    k_an_radii<<<grid_cap(h->n), kThreads, 0, h->st>>>(pos, h->n, centre, L, hL, radii, h->an_cnt);
 }
 
 }  // namespace
@@ -1093,6 +1127,80 @@ int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax) 
 }
 
 const char* pm3d_kernel_names(void) { return kKernelNames; }
+
+int pm3d_set_analysis_sets(void* handle, const int32_t* idx, const int64_t* offsets, int nsets) {
+    if (!handle || !idx || !offsets || nsets < 1) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    if (offsets[0] != 0) return fail(h, PM3D_EINVAL, "offsets[0] must be 0");
+    for (int s = 0; s < nsets; ++s)
+        if (offsets[s + 1] <= offsets[s]) return fail(h, PM3D_EINVAL, "every analysis set must be non-empty");
+    const long long tot = offsets[nsets];
+    for (long long j = 0; j < tot; ++j)
+        if (idx[j] < 0 || idx[j] >= h->n) return fail(h, PM3D_EINVAL, "analysis set index out of range");
+    HCHK(h, hipStreamSynchronize(h->st));  // queued analysis work may still read the old sets
+    void* old[] = {h->an_idx, h->an_off, h->an_buf, h->an_cnt};
+    for (void* b : old)
+        if (b) HCHK(h, hipFree(b));
+    h->an_idx = nullptr;
+    h->an_off = nullptr;
+    h->an_buf = nullptr;
+    h->an_cnt = nullptr;
+    h->an_nsets = 0;
+    const int ns = nsets - 1;
+    int rc;
+    if ((rc = dalloc(h, &h->an_idx, (size_t)tot)) || (rc = dalloc(h, &h->an_off, (size_t)nsets + 1)) ||
+        (rc = dalloc(h, &h->an_buf, 4 + 9 * (size_t)ns + (size_t)kAnBlocks * 9 * ns + 1)) ||
+        (rc = dalloc(h, &h->an_cnt, (size_t)kAnMaxRadii)))
+        return rc;
+    HCHK(h, hipMemcpy(h->an_idx, idx, (size_t)tot * sizeof(int), hipMemcpyHostToDevice));
+    static_assert(sizeof(long long) == sizeof(int64_t), "offsets");
+    HCHK(h, hipMemcpy(h->an_off, offsets, ((size_t)nsets + 1) * sizeof(long long), hipMemcpyHostToDevice));
+    h->an_nsets = nsets;
+    h->an_off_host1 = offsets[1];
+    return 0;
+}
+
+
+int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, double* moments, int64_t* counts,
+                 double* rho_max) {
+    if (!handle || nradii < 0 || nradii > kAnMaxRadii || (nradii > 0 && (!radii || !counts))) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    if (h->an_nsets < 1) return fail(h, PM3D_ESTATE, "no analysis sets (pm3d_set_analysis_sets)");
+    if (!h->state_set) return fail(h, PM3D_ESTATE, "no particle state");
+    AnRadii rr{};
+    rr.nr = nradii;
+    for (int k = 0; k < nradii; ++k) rr.r[k] = radii[k];
+    if (h->mixed)
+        launch_analyse(h, AnPosMx{h->cell, h->off, h->n, h->dx, h->L}, rr);
+    else
+        launch_analyse(h, AnPosF64{h->x, h->n}, rr);
+    int rc = check_launch(h, "analyse");
+    if (rc) return rc;
+    if (rho_max) {
+        // maximum of a fresh deposit of the state, into the gradient spectra (free between steps), so
+        // the density mesh of the last step stays readable
+        unsigned long long* scratch = (unsigned long long*)h->gspec;
+        if ((rc = deposit_into(h, scratch))) return rc;
+        const unsigned nb = (unsigned)std::min<size_t>(1024, h->bmax_len);
+        if (h->wide)
+            k_an_mesh_max<true><<<nb, kThreads, 0, h->st>>>(scratch, h->M, h->qscale, h->bmax);
+        else
+            k_an_mesh_max<false><<<nb, kThreads, 0, h->st>>>(scratch, h->M, h->qscale, h->bmax);
+        k_an_max_reduce<<<1, kThreads, 0, h->st>>>(h->bmax, (int)nb, h->an_buf + 3);
+        if ((rc = check_launch(h, "analyse rho max"))) return rc;
+    }
+    const int ns = h->an_nsets - 1;
+    std::vector<double> hb(4 + 9 * (size_t)ns);
+    unsigned long long hc[kAnMaxRadii];
+    HCHK(h, hipMemcpyAsync(hb.data(), h->an_buf, hb.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipMemcpyAsync(hc, h->an_cnt, sizeof(hc), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipStreamSynchronize(h->st));
+    if (centre) std::memcpy(centre, hb.data(), 3 * sizeof(double));
+    if (moments && ns > 0) std::memcpy(moments, hb.data() + 4, 9 * (size_t)ns * sizeof(double));
+    for (int k = 0; k < nradii; ++k) counts[k] = (int64_t)hc[k];
+    if (rho_max) *rho_max = hb[3];
+    return 0;
+}
 
 int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax) {
     if (!handle || !variants || nmax < 0) return PM3D_EINVAL;

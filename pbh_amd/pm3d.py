@@ -358,6 +358,44 @@ class PM3D:
     def set_timing(self, on: bool):
         check(self._L.pm3d_set_timing(self._h, 1 if on else 0), self._h)
 
+    def analysis_sets(self, fracs=(0.1, 0.2, 0.3)) -> list:
+        """Index sets of analyse (pm3d.py:136-146), uploaded to the device once per fracs: the 27
+        innermost particles by r_ell_q, then per fraction r_ell_q < f*sqrt(6) (the 30 innermost
+        when fewer than 30), each in ascending particle order as the reference's boolean masks."""
+        key = tuple(float(f) for f in fracs)
+        if getattr(self, "_an_key", None) == key:
+            return self._an_sets
+        if not hasattr(self, "r_ell_q"):
+            raise RuntimeError("analyse needs the Lagrangian radii r_ell_q of make_ic")
+        rm = np.sqrt(6.0)
+        order = np.argsort(self.r_ell_q)
+        sets = [np.sort(order[:27])]
+        for f in key:
+            sel = np.flatnonzero(self.r_ell_q < f * rm)
+            if len(sel) < 30:
+                sel = np.sort(order[:30])
+            sets.append(sel)
+        idx = np.ascontiguousarray(np.concatenate(sets), dtype=np.int32)
+        offs = np.zeros(len(sets) + 1, np.int64)
+        offs[1:] = np.cumsum([len(t) for t in sets])
+        check(self._L.pm3d_set_analysis_sets(self._h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                             iptr(offs), len(sets)), self._h)
+        self._an_key, self._an_sets = key, sets
+        return sets
+
+    def device_analyse(self, radii):
+        """Centre, per-set mean and outer-product sums, counts within radii and the density
+        maximum of the current state (pm3d_analyse)."""
+        radii = np.ascontiguousarray(radii, dtype=np.float64)
+        ns = len(self._an_sets) - 1
+        centre = np.zeros(3)
+        mom = np.zeros((max(ns, 1), 9))
+        cnt = np.zeros(max(len(radii), 1), np.int64)
+        rmax = ctypes.c_double()
+        check(self._L.pm3d_analyse(self._h, dptr(radii), len(radii), dptr(centre), dptr(mom), iptr(cnt),
+                                   ctypes.byref(rmax)), self._h)
+        return centre, mom[:ns], cnt[: len(radii)], rmax.value
+
     def fft_tiles(self) -> dict:
         """FFT tile variant per pass chosen at creation (0: 16 KB buffers, radix <= 4; 1: 32 KB, radix 8/16)."""
         v = np.zeros(5, np.int32)
@@ -373,43 +411,34 @@ class PM3D:
 
 
 def analyse(pm: PM3D, a: float, fracs=(0.1, 0.2, 0.3), radii_cells=(1.0, 2.0, 4.0, 8.0)):
-    """Inner Lagrangian-set axes and enclosed-mass compaction (pm3d.py:132-159).
+    """Inner Lagrangian-set axes and enclosed-mass compaction (pm3d.py:132-159), on the device.
 
-    Host diagnostic, as in the reference: positions are copied from the device once; the density
-    maximum comes from the device deposit.
+    The index sets (the 27 innermost particles by r_ell_q, and r_ell_q < f*sqrt(6) per fraction,
+    or the 30 innermost when a set has fewer than 30) are computed on the host from r_ell_q once
+    and kept on the device; the centre, set means and second moments, enclosed counts and the
+    density maximum are reductions over the particle state in HBM.  Only the 3 x 3 eigenvalue
+    problems run on the host.
     """
-    L = pm.L
-    x = pm.x
     out = {"a": a}
-    rm = np.sqrt(6.0)
-    order = np.argsort(pm.r_ell_q)
-    core = np.zeros(len(x), bool)
-    core[order[:27]] = True
-    xs = x[core]
-    rel = (xs - xs[0] + L / 2) % L - L / 2
-    centre = (xs[0] + rel.mean(axis=0)) % L
-    rel_all = (x - centre + L / 2) % L - L / 2
-    for f in fracs:
-        sel = pm.r_ell_q < f * rm
-        if sel.sum() < 30:
-            sel = np.zeros(len(x), bool)
-            sel[order[:30]] = True
-        r = rel_all[sel]
-        r = r - r.mean(axis=0)
-        inertia = r.T @ r / len(r)
+    sets = pm.analysis_sets(tuple(fracs))
+    radii = np.array([nc * pm.dx for nc in radii_cells], dtype=np.float64)
+    _, mom, cnt, rho_max = pm.device_analyse(radii)
+    for k, f in enumerate(fracs):
+        m = len(sets[k + 1])
+        xx, xy, xz, yy, yz, zz = mom[k][3:]
+        inertia = np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]) / m
         ev = np.sqrt(np.maximum(np.linalg.eigvalsh(inertia), 0.0)) * np.sqrt(5.0)
-        M = pm.m_part * sel.sum()
+        M = pm.m_part * np.int64(m)
         axes_phys = a * ev[::-1]
         R_hoop = np.sqrt(0.5 * (axes_phys[0] ** 2 + axes_phys[1] ** 2))
         out[f"axes_{f}"] = axes_phys.tolist()
         out[f"axes_com_{f}"] = ev[::-1].tolist()
         out[f"M_{f}"] = M
         out[f"hoop_{f}"] = 2 * M / max(R_hoop, 1e-30)
-    r_com = np.sqrt(np.sum(rel_all ** 2, axis=1))
-    for nc in radii_cells:
+    for nc, c in zip(radii_cells, cnt):
         R = a * nc * pm.dx
-        M = pm.m_part * np.count_nonzero(r_com < nc * pm.dx)
+        M = pm.m_part * int(c)
         out[f"m_in_{nc:g}c"] = M
         out[f"C_{nc:g}c"] = 2 * M / R
-    out["rho_max_over_mean"] = float(np.max(pm.deposit(x)) / (pm.Np ** 3 / pm.N ** 3))
+    out["rho_max_over_mean"] = float(rho_max / (pm.Np ** 3 / pm.N ** 3))
     return out
