@@ -67,6 +67,9 @@ int pm3d_destroy(void* handle);
 const char* pm3d_last_error(void* handle);  /* handle may be NULL: the last create error */
 int pm3d_synchronize(void* handle);         /* waits for the stream; returns the sticky bad count */
 int64_t pm3d_device_bytes(void* handle);    /* HBM held by the handle */
+/* Name, gcnArchName and total memory of the handle's device, for run records (the reference's
+ * provenance record, provenance.py:52-64, names the host CPU and compiler). */
+int pm3d_device_info(void* handle, char* name, int name_len, char* arch, int arch_len, int64_t* total_bytes);
 
 /* State transfer (the reference keeps pm.x / pm.p as numpy attributes, pm3d.py:67-68,124). */
 int pm3d_set_state(void* handle, const double* x, const double* p, int64_t n);

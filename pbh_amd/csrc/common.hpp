@@ -118,6 +118,31 @@ __device__ __forceinline__ void cic_axis_fast(double X, double dx, int N, int& l
 // v in [-N, 2N) wrapped into [0, N) without an integer division.
 __device__ __forceinline__ int wrapN(int v, int N) { return v < 0 ? v + N : (v >= N ? v - N : v); }
 
+// Workgroups are dispatched to the 8 XCDs of gfx950 round-robin by linear workgroup id, and each XCD
+// has its own L2.  Neighbouring tiles share cache lines (pencil segments shorter than a line, CIC
+// corners of adjacent particle planes, the one-site ring of deposit columns); these maps send them
+// to one XCD so that they meet in one L2:
+//   xcd_block: every XCD runs one contiguous range of logical ids;
+//   xcd_map mode 2: runs of c consecutive logical ids are dealt round-robin to the XCDs (needs
+//   G % (8c) == 0), so all XCDs stream from neighbouring addresses at any moment.
+// Both are bijections of [0, G), so results never depend on them.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) {
+    const unsigned q = G / kXcds, r = G % kXcds;
+    const unsigned x = b % kXcds, k = b / kXcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+struct XMap {
+    int mode;  // 0 identity, 1 xcd_block, 2 runs of c round-robin over the XCDs
+    unsigned c;
+};
+__device__ __forceinline__ unsigned xcd_map(unsigned b, unsigned G, XMap m) {
+    if (m.mode == 0) return b;
+    if (m.mode == 1) return xcd_block(b, G);
+    const unsigned x = b % kXcds, k = b / kXcds;
+    return (k / m.c) * (kXcds * m.c) + x * m.c + (k % m.c);
+}
+
 // Exact conversion of an integral double in [0, 2^63) to uint64 with two 32-bit conversions.
 __device__ __forceinline__ unsigned long long u64_of_integral(double q) {
     const unsigned int hi = (unsigned int)(q * 0x1p-32);

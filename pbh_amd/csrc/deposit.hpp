@@ -81,7 +81,7 @@ __device__ __forceinline__ int site_offset(int lo, int site, int N) {
 
 template <class Src, int BX, int BY, int N, bool WIDE, int NT>
 __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long* __restrict__ mesh,
-                                                    int* __restrict__ far, unsigned int* __restrict__ farcnt) {
+                                                    int* __restrict__ far, unsigned int* __restrict__ farcnt, XMap xm) {
     constexpr int NCELL = BX * BY * N;
     constexpr int EX = BX + 2, EY = BY + 2, NSITE = EX * EY * N;
     // WIDE: the 128-bit cell is accumulated as two 64-bit sums of the low and high 32-bit halves of
@@ -96,8 +96,9 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
     if (threadIdx.x == 0) nfar = 0u;
     __syncthreads();
     constexpr int NBY = N / BY;
-    const int x0 = (blockIdx.x / NBY) * BX, y0 = (blockIdx.x % NBY) * BY;
-    int* myfar = far + (size_t)blockIdx.x * (BX * BY * N);
+    const int col = (int)xcd_map(blockIdx.x, gridDim.x, xm);  // column id, also this column's far segment
+    const int x0 = (col / NBY) * BX, y0 = (col % NBY) * BY;
+    int* myfar = far + (size_t)col * (BX * BY * N);
     constexpr int U = 4;  // sites whose loads are issued together
     for (int t0 = threadIdx.x; t0 < NSITE; t0 += NT * U) {
         typename Src::Raw raw[U];
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) farcnt[blockIdx.x] = nfar;
+    if (threadIdx.x == 0) farcnt[col] = nfar;
     for (int t = threadIdx.x; t < NCELL; t += NT) {
         const int rz = t % N, ry = (t / N) % BY, rx = t / (N * BY);
         const size_t cell = ((size_t)(x0 + rx) * N + (y0 + ry)) * N + rz;

@@ -212,7 +212,7 @@ template <typename R, int N, bool INV, int V>
 __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
                                                     size_t cs, size_t cn, const cplx<R>* __restrict__ tw,
                                                     const double* __restrict__ bmax, int nbmax,
-                                                    unsigned long long* __restrict__ slot) {
+                                                    unsigned long long* __restrict__ slot, XMap xm) {
     constexpr int P = Plan<R, N, V>::PY, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
     __shared__ cplx<R> A[N * PP], B[N * PP];
     if (nbmax > 0 && blockIdx.x == gridDim.x - 1 && blockIdx.y == 0) {
@@ -221,9 +221,10 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
         m = block_max_nonneg(m);
         if (threadIdx.x == 0) *slot = (unsigned long long)__double_as_longlong(m);
     }
-    spec += blockIdx.y * cs;
-    nyq += blockIdx.y * cn;
-    const int b = blockIdx.x;
+    const unsigned lin = xcd_map(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y, xm);
+    const int b = (int)(lin % gridDim.x);
+    spec += (lin / gridDim.x) * cs;
+    nyq += (lin / gridDim.x) * cn;
     if (b < NBM) {
         const int x = b / (NH / P), col0 = (b % (NH / P)) * P;
         cplx<R>* base = spec + (size_t)x * N * NH + col0;
@@ -270,7 +271,8 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
                                                      cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
                                                      size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
                                                      double kval, double s, double sm2, int smooth_on,
-                                                     const DevSched* __restrict__ sch, const DevStep* __restrict__ tab) {
+                                                     const DevSched* __restrict__ sch, const DevStep* __restrict__ tab,
+                                                     XMap xm) {
     if (MODE == 2 && sch) {  // Green's function factor of the scheduled step (last entry past the end)
         const int k = sch->counter < sch->nsteps ? sch->counter : sch->nsteps - 1;
         s = tab[k].s;
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
     constexpr int P = Plan<R, N, V>::PX, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
     constexpr int EPT = (N * P + kThreads - 1) / kThreads;
     __shared__ cplx<R> A[N * PP], B[N * PP];
-    const int b = blockIdx.x;
+    const int b = (int)xcd_map(blockIdx.x, gridDim.x, xm);
     const bool main = b < NBM;
     int y = 0, col0 = 0;
     if (main) {

@@ -72,8 +72,8 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
 template <bool KICK>
 __global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__ x, long long n, int N, double dx,
                                                       const double* __restrict__ grad, double* __restrict__ acc,
-                                                      double* __restrict__ p, double c1, double den) {
-    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+                                                      double* __restrict__ p, double c1, double den, XMap xm) {
+    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
     if (i >= n) return;
     int lo[3], hi[3];
     double f[3];

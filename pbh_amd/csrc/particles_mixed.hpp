@@ -79,8 +79,8 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic_mx(MxPos P, int N, 
 
 // CIC gather of the interleaved FP32 gradient mesh into acc (SoA).
 __global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const float* __restrict__ grad,
-                                                         float* __restrict__ acc) {
-    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+                                                         float* __restrict__ acc, XMap xm) {
+    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
     if (i >= P.n) return;
     int lo[3];
     float f[3];

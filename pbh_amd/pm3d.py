@@ -354,6 +354,14 @@ class PM3D:
     def device_bytes(self) -> int:
         return int(self._L.pm3d_device_bytes(self._h))
 
+    def device_info(self) -> dict:
+        """Device name, architecture and total memory of the handle's GPU."""
+        name = ctypes.create_string_buffer(256)
+        arch = ctypes.create_string_buffer(256)
+        tot = ctypes.c_int64()
+        check(self._L.pm3d_device_info(self._h, name, 256, arch, 256, ctypes.byref(tot)), self._h)
+        return {"name": name.value.decode(), "arch": arch.value.decode(), "total_bytes": int(tot.value)}
+
     # -- per-kernel timing (hipEvents on the handle's stream)
     def set_timing(self, on: bool):
         check(self._L.pm3d_set_timing(self._h, 1 if on else 0), self._h)

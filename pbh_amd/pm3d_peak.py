@@ -5,7 +5,9 @@ JSON output ({out}/{tag}.json with args, rows, rm, t_H, tC0, dx, m_part).
     python -m pbh_amd.pm3d_peak --mu 0.1 --e 0 --p 0 --N 128 --Np 64 --L 8 --tend 40 [--out runs/pm3d]
 
 Additional flags (not in the reference): --precision {fp64,mixed}; --collapse stops at the first
-shell crossing (min det(dx/dq) <= 0, Np lattice) and records t_c in the JSON.
+shell crossing (min det(dx/dq) <= 0, Np lattice) and records t_c in the JSON.  Every run also
+writes {out}/{tag}_provenance.json (pbh_amd.provenance: git commit, compiler and flags, host, GPU,
+parameters, seed, times, peak RSS, steps, particle-steps/s, t_c).
 """
 from __future__ import annotations
 
@@ -18,6 +20,7 @@ import numpy as np
 
 from .ltb import LTBGaussian, profile_table
 from .pm3d import PM3D, T_I, analyse
+from .provenance import RunRecord
 
 
 def parser():
@@ -56,6 +59,7 @@ def run_tag(args) -> str:
 def main(argv=None):
     args = parser().parse_args(argv)
     Path(args.out).mkdir(parents=True, exist_ok=True)
+    record = RunRecord(args.out, vars(args), args.seed, name=f"{run_tag(args)}_provenance.json")
     table = profile_table(args.profile)
     ref = LTBGaussian(args.mu)
     rm, t_H = ref.r_m, ref.t_H
@@ -110,6 +114,10 @@ def main(argv=None):
         out["collapse"] = collapse
     with open(f"{args.out}/{tag}.json", "w") as fh:
         json.dump(out, fh)
+    loop_s = time.time() - t0
+    record.finish(pm, extra={"steps": step, "a_end": a, "loop_wall_s": loop_s,
+                             "particle_steps_per_s": args.Np ** 3 * step / max(loop_s, 1e-9),
+                             "collapse": collapse, "output": f"{args.out}/{tag}.json"})
     print("done", f"{time.time() - t0:.0f}s")
     return out
 

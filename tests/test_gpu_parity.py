@@ -287,6 +287,8 @@ def test_driver_rows_match_oracle_loop(tmp_path):
         for key in ("axes_com_0.2", "hoop_0.1", "m_in_2c", "rho_max_over_mean", "delta_max"):
             np.testing.assert_allclose(np.asarray(r_gpu[key]), np.asarray(r_ref[key]), rtol=1e-8, err_msg=key)
     assert (tmp_path / "mu0.3_e0.1_p0_N16_Np8_L8_sm2.json").exists()
+    rec = json.loads((tmp_path / "mu0.3_e0.1_p0_N16_Np8_L8_sm2_provenance.json").read_text())
+    assert rec["gpu"]["arch"].startswith("gfx950") and rec["steps"] == rows[-1]["step"]
 
 
 @pytest.mark.parametrize("N,prec", [(128, "fp64"), (128, "mixed"), (96, "fp64"), (256, "mixed")])

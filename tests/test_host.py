@@ -73,3 +73,18 @@ def test_driver_flags_and_tag_match_reference_driver():
     assert run_tag(a) == "mu0.1_e0_p0_N128_Np64_L8_sm2"
     b = parser().parse_args(["--tail", "0.05", "--seed", "3", "--mu", "0.25"])
     assert run_tag(b) == "mu0.25_e0_p0_N128_Np64_L8_sm2_tail0.05_k2-10_s3"
+
+
+def test_run_record_has_reference_provenance_fields(tmp_path):
+    """The run record carries every field of the reference's provenance record
+    (provenance.py:48-76) plus the solver extras."""
+    import json
+    from pbh_amd.provenance import RunRecord
+    rec = RunRecord(tmp_path, {"mu": 0.1, "N": 16}, 3, name="r_provenance.json")
+    out = rec.finish(extra={"collapse": {"t_c": np.float64(1.5)}, "steps": np.int64(7)})
+    on_disk = json.loads((tmp_path / "r_provenance.json").read_text())
+    for key in ("git", "python", "numpy", "compiler", "compile_flags", "host", "cpu", "n_cpu", "mpi", "params",
+                "seed", "start_utc", "end_utc", "wall_time_s", "peak_rss_MB"):
+        assert key in on_disk, key
+    assert on_disk["collapse"]["t_c"] == 1.5 and on_disk["steps"] == 7 and out["seed"] == 3
+    assert "--offload-arch=gfx950" in on_disk["compile_flags"]
