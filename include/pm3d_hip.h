@@ -134,10 +134,10 @@ int pm3d_set_analysis_sets(void* handle, const int32_t* idx, const int64_t* offs
 int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, double* moments, int64_t* counts,
                  double* rho_max);
 
-/* FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z), two bits: bit 0 = 32 KB rather than
- * 16 KB LDS buffers (more pencils per workgroup; same bits), bit 1 = radix 8 (FP64) / 16 (FP32)
- * rather than 4 (a different rounding sequence).  pm3d_create times both buffer sizes and keeps
- * the faster (cached per mesh size and precision); the radix is fixed per pass.  Environment
+/* FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z), two bits: bit 0 = pencils per
+ * workgroup sized for 32 KB rather than 16 KB LDS buffers, bit 1 = one buffer transformed in place
+ * rather than two ping-pong buffers.  Every variant gives the same bits; pm3d_create times all four
+ * per pass and keeps the fastest (cached per mesh size and precision).  Environment
  * PM3D_FFT_TILES = one digit 0-3, or five digits, forces the variants.  Returns the number of
  * passes.  No reference counterpart (numpy's FFT has no tiles). */
 int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax);

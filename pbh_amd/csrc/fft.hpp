@@ -27,19 +27,32 @@ constexpr int pow2_floor(int v) { return v >= 2 ? 2 * pow2_floor(v / 2) : 1; }
 constexpr int pow2_dividing(int cap, int total) {  // largest power of two <= cap dividing total
     return (cap <= 1) ? 1 : (total % cap == 0 ? cap : pow2_dividing(cap / 2, total));
 }
-// Tile configuration V of a pass:
-//   bit 0: ping-pong buffers of ~16 KB (0: more resident workgroups) or ~32 KB (1: more pencils
-//          per workgroup).  The buffer size changes only how many pencils a workgroup holds, so
-//          both give the same bits.
-//   bit 1: largest radix 4 (0) or 8 in FP64 / 16 in FP32 (1): fewer LDS round trips per pencil,
-//          but a different (equally accurate) rounding sequence.
+// Pass kinds, for the radix table.
+enum FftPassKind { PK_R2C = 0, PK_YFWD, PK_X, PK_YINV, PK_C2R };
+
+// log2 of the largest radix of each pass (MI355X timings, us per launch at 128^3 FP64 / 256^3 FP32,
+// radix 4 -> radix 8 (FP64) or 16 (FP32)): FP64 x 25.9 -> 24.9, y inv 24.2 -> 23.7, the other
+// passes equal within 1 %; FP32 r2c 56.9 -> 51.9, y fwd 45.0 -> 44.2, y inv 122 -> 114, c2r 171 ->
+// 148, but x 99 -> 116 (the Poisson x pass keeps its per-thread spectrum in registers, which radix
+// 16 crowds).  The radix fixes the rounding sequence, so it is a constant, not a tuning choice.
+template <typename R, int PK>
+constexpr int radix_l2() {
+    if constexpr (sizeof(R) == 8) return (PK == PK_X || PK == PK_YINV) ? 3 : 2;
+    else return PK == PK_X ? 2 : 4;
+}
+
+// Tile configuration V of a pass (every V computes the same bits, so pm3d.hip picks by timing):
+//   bit 0: pencils per workgroup sized for ~16 KB (0) or ~32 KB (1) per LDS buffer;
+//   bit 1: two ping-pong buffers (0) or one buffer transformed in place (1: half the LDS, so twice
+//          the resident workgroups, at the cost of a second barrier per radix pass).
 template <typename R, int N, int V> struct Plan {
     static constexpr int EB = ((V & 1) ? 32768 : 16384) / (2 * (int)sizeof(R));  // complex elements per buffer
-    static constexpr int ML2 = (V & 2) ? (sizeof(R) == 4 ? 4 : 3) : 2;           // log2 of the largest radix
+    static constexpr bool INPLACE = (V & 2) != 0;
     static constexpr int PZ = pow2_dividing(pow2_floor(cmin(2 * EB / N, N * N)), N * N);  // r2c rows
     static constexpr int PY = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // y columns
     static constexpr int PX = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // x columns
     static constexpr int PC = pow2_dividing(pow2_floor(cmin(((V & 1) ? EB / 2 : EB) / N, N * N)), N * N);  // c2r rows
+    static constexpr int NBUF = INPLACE ? 1 : 2;
 };
 
 // In-register DFT of size R in {2, 3, 4, 8, 16} (forward: exp(-2 pi i km/R); INV: conjugate).
@@ -131,14 +144,53 @@ __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, 
     __syncthreads();
 }
 
+// The same pass on one buffer: every thread reads its RADIX-point inputs, applies the twiddles and
+// the DFT in registers, and writes back after a barrier.
+template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
+__device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, int Ns, const cplx<R>* __restrict__ tw) {
+    constexpr int PP = P + 1, M = NC / RADIX, T = P * M, IT = (T + kThreads - 1) / kThreads;
+    const int s = TWS * (NC / (RADIX * Ns));
+    cplx<R> v[IT][RADIX];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int t = threadIdx.x + it * kThreads;
+        if (T % kThreads == 0 || t < T) {
+            const int c = t % P, j = t / P, k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) v[it][r] = buf[(j + r * M) * PP + c];
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < RADIX; ++r) {
+                    cplx<R> w = tw[r * k * s];
+                    if (INV) w = cconj(w);
+                    v[it][r] = cmul(v[it][r], w);
+                }
+            }
+            dft_reg<R, RADIX, INV>(v[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int t = threadIdx.x + it * kThreads;
+        if (T % kThreads == 0 || t < T) {
+            const int c = t % P, j = t / P, k = j % Ns;
+            const int dd = (j / Ns) * Ns * RADIX + k;
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) buf[(dd + r * Ns) * PP + c] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
 constexpr int count_factor(int n, int f) { return (n % f == 0) ? 1 + count_factor(n / f, f) : 0; }
 
 // In-LDS Stockham FFT of P pencils of NC points (NC = 2^a 3^b, unnormalised; forward =
 // exp(-2 pi i jk/NC)).  Power-of-two passes of radix 2^ML2 (then one smaller power of two for the
 // rest), then radix-3 passes; each pass is one LDS round trip.  Input in `a`, result in the
-// returned buffer.  Caller synchronises before the call; the call ends with a barrier.
-// tw[k] = exp(-2 pi i k / (NC*TWS)).
-template <typename R, int NC, int P, bool INV, int TWS, int ML2, int NS = 1, int REM2 = count_factor(NC, 2),
+// returned buffer (always `a` when IP: in place, `b` unused).  Caller synchronises before the call;
+// the call ends with a barrier.  tw[k] = exp(-2 pi i k / (NC*TWS)).
+template <typename R, int NC, int P, bool INV, int TWS, int ML2, bool IP, int NS = 1, int REM2 = count_factor(NC, 2),
           int REM3 = count_factor(NC, 3)>
 __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R>* __restrict__ tw) {
     if constexpr (REM2 == 0 && REM3 == 0) {
@@ -146,8 +198,14 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R
     } else {
         constexpr int L2 = REM2 >= ML2 ? ML2 : REM2;
         constexpr int RADIX = REM2 > 0 ? (1 << L2) : 3;
-        stockham_pass<R, NC, P, INV, TWS, RADIX>(a, b, NS, tw);
-        return lds_fft<R, NC, P, INV, TWS, ML2, NS * RADIX, (REM2 > 0 ? REM2 - L2 : 0), (REM2 > 0 ? REM3 : REM3 - 1)>(b, a, tw);
+        constexpr int R2 = REM2 > 0 ? REM2 - L2 : 0, R3 = REM2 > 0 ? REM3 : REM3 - 1;
+        if constexpr (IP) {
+            stockham_pass_ip<R, NC, P, INV, TWS, RADIX>(a, NS, tw);
+            return lds_fft<R, NC, P, INV, TWS, ML2, IP, NS * RADIX, R2, R3>(a, b, tw);
+        } else {
+            stockham_pass<R, NC, P, INV, TWS, RADIX>(a, b, NS, tw);
+            return lds_fft<R, NC, P, INV, TWS, ML2, IP, NS * RADIX, R2, R3>(b, a, tw);
+        }
     }
 }
 
@@ -163,7 +221,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
                                                     double* __restrict__ bmax, unsigned long long* __restrict__ nfar,
                                                     double qscale, double nmean) {
     constexpr int NC = N / 2, P = Plan<R, N, V>::PZ, PP = P + 1;
-    __shared__ cplx<R> A[NC * PP], B[NC * PP];
+    __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
     double lmax = 0.0;
     for (int t = threadIdx.x; t < P * NC; t += kThreads) {
@@ -190,7 +248,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
         }
     }
     __syncthreads();
-    const cplx<R>* Z = lds_fft<R, NC, P, false, 2, Plan<R, N, V>::ML2>(A, B, tw);
+    const cplx<R>* Z = lds_fft<R, NC, P, false, 2, radix_l2<R, PK_R2C>(), Plan<R, N, V>::INPLACE>(A, B, tw);
     for (int t = threadIdx.x; t < P * NC; t += kThreads) {
         const int c = t / NC, k = t - c * NC;
         const cplx<R> zk = Z[k * PP + c];
@@ -214,7 +272,9 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
                                                     const double* __restrict__ bmax, int nbmax,
                                                     unsigned long long* __restrict__ slot, XMap xm) {
     constexpr int P = Plan<R, N, V>::PY, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
-    __shared__ cplx<R> A[N * PP], B[N * PP];
+    constexpr int ML2 = radix_l2<R, INV ? PK_YINV : PK_YFWD>();
+    constexpr bool IP = Plan<R, N, V>::INPLACE;
+    __shared__ cplx<R> A[N * PP], B[IP ? 1 : N * PP];
     if (nbmax > 0 && blockIdx.x == gridDim.x - 1 && blockIdx.y == 0) {
         double m = 0.0;
         for (int t = threadIdx.x; t < nbmax; t += kThreads) m = fmax(m, bmax[t]);
@@ -233,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
             A[j * PP + c] = base[(size_t)j * NH + c];
         }
         __syncthreads();
-        const cplx<R>* res = lds_fft<R, N, P, INV, 1, Plan<R, N, V>::ML2>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, INV, 1, ML2, IP>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;
             base[(size_t)j * NH + c] = res[j * PP + c];
@@ -245,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
             A[j * PP + c] = base[(size_t)c * N + j];
         }
         __syncthreads();
-        const cplx<R>* res = lds_fft<R, N, P, INV, 1, Plan<R, N, V>::ML2>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, INV, 1, ML2, IP>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int j = t % N, c = t / N;
             base[(size_t)c * N + j] = res[j * PP + c];
@@ -281,7 +341,9 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
     }
     constexpr int P = Plan<R, N, V>::PX, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
     constexpr int EPT = (N * P + kThreads - 1) / kThreads;
-    __shared__ cplx<R> A[N * PP], B[N * PP];
+    constexpr int ML2 = radix_l2<R, PK_X>();
+    constexpr bool IP = Plan<R, N, V>::INPLACE;
+    __shared__ cplx<R> A[N * PP], B[IP ? 1 : N * PP];
     const int b = (int)xcd_map(blockIdx.x, gridDim.x, xm);
     const bool main = b < NBM;
     int y = 0, col0 = 0;
@@ -304,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
     if constexpr (MODE == 3) {
         // Zel'dovich displacement: psi_c = 1j*k_c*dk/k2 with dk(0) = 0, evaluated as numpy does
         // ((0 + 1j*k_c) * dk, then complex/real as a multiply by 1/k2), then inverse x FFTs
-        const cplx<R>* res = lds_fft<R, N, P, false, 1, Plan<R, N, V>::ML2>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, false, 1, ML2, IP>(A, B, tw);
         cplx<R> dkr[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
@@ -332,7 +394,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
                 }
             }
             __syncthreads();
-            const cplx<R>* r2 = lds_fft<R, N, P, true, 1, Plan<R, N, V>::ML2>(A, B, tw);
+            const cplx<R>* r2 = lds_fft<R, N, P, true, 1, ML2, IP>(A, B, tw);
             cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
             for (int t = threadIdx.x; t < N * P; t += kThreads) {
                 const int c = t % P, j = t / P;
@@ -342,14 +404,14 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
         }
         return;
     } else if constexpr (MODE < 2) {
-        const cplx<R>* res = lds_fft<R, N, P, MODE == 1, 1, Plan<R, N, V>::ML2>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, MODE == 1, 1, ML2, IP>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;
             src[off(j, c)] = res[j * PP + c];
         }
         return;
     } else {
-        const cplx<R>* res = lds_fft<R, N, P, false, 1, Plan<R, N, V>::ML2>(A, B, tw);
+        const cplx<R>* res = lds_fft<R, N, P, false, 1, ML2, IP>(A, B, tw);
         cplx<R> ph[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
@@ -387,7 +449,7 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
                 }
             }
             __syncthreads();
-            const cplx<R>* r2 = lds_fft<R, N, P, true, 1, Plan<R, N, V>::ML2>(A, B, tw);
+            const cplx<R>* r2 = lds_fft<R, N, P, true, 1, ML2, IP>(A, B, tw);
             cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
             for (int t = threadIdx.x; t < N * P; t += kThreads) {
                 const int c = t % P, j = t / P;
@@ -407,7 +469,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ 
                                                     size_t gcs, size_t gcn, R* __restrict__ out,
                                                     const cplx<R>* __restrict__ tw, R scale) {
     constexpr int NC = N / 2, P = Plan<R, N, V>::PC, PP = P + 1;
-    __shared__ cplx<R> A[NC * PP], B[NC * PP];
+    __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
     __shared__ R O[P * N * NCOMP];
     const size_t row0 = (size_t)blockIdx.x * P;
     for (int comp = 0; comp < NCOMP; ++comp) {
@@ -425,7 +487,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ 
             A[k * PP + c] = cplx<R>{er - oi, ei + orr};
         }
         __syncthreads();
-        const cplx<R>* z = lds_fft<R, NC, P, true, 2, Plan<R, N, V>::ML2>(A, B, tw);
+        const cplx<R>* z = lds_fft<R, NC, P, true, 2, radix_l2<R, PK_C2R>(), Plan<R, N, V>::INPLACE>(A, B, tw);
         for (int t = threadIdx.x; t < P * NC; t += kThreads) {
             const int c = t / NC, m = t - c * NC;
             const cplx<R> v = z[m * PP + c];

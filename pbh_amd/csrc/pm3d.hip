@@ -287,7 +287,7 @@ struct Fft {
     // r2c z + forward y on h->spec.  fixed: input is the deposit mesh.
     static void forward(Handle* h, const void* in, bool fixed) {
         PM3D_V4(h->fftv[FP_R2C], r2c, h, in, fixed);
-        const int nb = fixed ? (int)((h->fftv[FP_R2C] & 1) ? r2c_grid<1>() : r2c_grid<0>()) : 0;
+        const int nb = fixed ? (int)((h->fftv[FP_R2C] & 1) ? r2c_grid<1>() : r2c_grid<0>()) : 0;  // PZ depends on bit 0 only
         PM3D_V4(h->fftv[FP_YFWD], yfwd, h, nb);
     }
 
@@ -366,16 +366,11 @@ int fft_launch_variant(Handle* h, int pass, int v) {
                     : fft_call<double>(h, 3, nullptr, false, pass, 0, 0, nullptr, v, nullptr);
 }
 
-// Picks the faster buffer size (Plan bit 0) of each FFT pass by timing both on this handle's
-// buffers (5 launches each after one warm-up, on zeroed buffers); the radix (Plan bit 1) is fixed
-// per precision and pass by kFftRadixBit, so the choice never changes the results.  The choice is
-// cached per (N, precision) for the process.  PM3D_FFT_TILES = one digit 0-3 forces that variant
-// for every pass, five digits one per pass (r2c_z, y fwd, x, y inv, c2r_z).
-// Radix table from MI355X timings (128^3 FP64, 256^3 FP32; us per launch, radix 4 -> 8/16):
-//   FP64: x 25.9 -> 24.9, y inv 24.2 -> 23.7, the others equal within 1%.
-//   FP32: r2c 56.9 -> 51.9, y fwd 45.0 -> 44.2, y inv 122 -> 114, c2r 171 -> 148; x 99 -> 116
-//   (the Poisson x pass keeps its per-thread spectrum in registers, which radix 16 crowds).
-constexpr int kFftRadixBit[2][FP_COUNT] = {{0, 0, 1, 1, 0}, {1, 1, 0, 1, 1}};
+// Picks the fastest tile variant of each FFT pass (fft.hpp Plan: pencils per workgroup x one or two
+// LDS buffers) by timing all four on this handle's buffers (5 launches each after one warm-up, on
+// zeroed buffers).  Every variant computes the same bits, so the choice never changes a result.
+// The choice is cached per (N, precision) for the process.  PM3D_FFT_TILES = one digit 0-3 forces
+// that variant for every pass, five digits one per pass (r2c_z, y fwd, x, y inv, c2r_z).
 std::mutex g_tune_mu;
 std::map<std::pair<int, bool>, std::array<int, FP_COUNT>> g_tuned;
 
@@ -409,19 +404,21 @@ int tune_fft(Handle* h) {
     int rc = 0;
     std::array<int, FP_COUNT> best{};
     for (int p = 0; p < FP_COUNT && !rc; ++p) {
-        float ms[2] = {0.f, 0.f};
-        const int rb = kFftRadixBit[h->mixed ? 1 : 0][p] << 1;
-        for (int v = 0; v < 2 && !rc; ++v) {
-            if ((rc = fft_launch_variant(h, p, rb | v))) break;
+        float ms[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int v = 0; v < 4 && !rc; ++v) {
+            if ((rc = fft_launch_variant(h, p, v))) break;
             if (hipEventRecord(e0, h->st) != hipSuccess) { rc = fail(h, PM3D_EHIP, "hipEventRecord failed"); break; }
-            for (int r = 0; r < 5 && !rc; ++r) rc = fft_launch_variant(h, p, rb | v);
+            for (int r = 0; r < 5 && !rc; ++r) rc = fft_launch_variant(h, p, v);
             if (rc) break;
             if (hipEventRecord(e1, h->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&ms[v], e0, e1) != hipSuccess)
                 rc = fail(h, PM3D_EHIP, "FFT tile timing failed");
             if (!rc) rc = check_launch(h, "FFT tile timing");
         }
-        best[p] = rb | (ms[1] < ms[0] ? 1 : 0);
+        int b = 0;
+        for (int v = 1; v < 4; ++v)
+            if (ms[v] < ms[b]) b = v;
+        best[p] = b;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);

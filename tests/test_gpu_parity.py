@@ -293,9 +293,9 @@ def test_driver_rows_match_oracle_loop(tmp_path):
 
 @pytest.mark.parametrize("N,prec", [(128, "fp64"), (128, "mixed"), (96, "fp64"), (256, "mixed")])
 def test_fft_tile_variants(PM3D, N, prec, monkeypatch):
-    """FFT tile configurations: the buffer size (variant bit 0) changes only how many pencils a
-    workgroup holds, so 0/1 and 2/3 give the same bits; the radix (bit 1) changes the rounding
-    sequence, so 0 and 2 agree to the mode's tolerance (FP64 1e-12 relative, mixed 1e-5)."""
+    """Every FFT tile configuration (pencils per workgroup x one or two LDS buffers) transforms
+    each pencil with the same operations, so a step's positions, momenta and density contrast
+    agree bit for bit across all four."""
     from pbh_amd import ic
     say("fft tiles", N, prec)
     spec = ic.PeakSpec(N=N, mu=0.3, sigma=(0.12, 0.11, 0.10))
@@ -311,11 +311,6 @@ def test_fft_tile_variants(PM3D, N, prec, monkeypatch):
             a, dm = pm.step(a, 0.02 * a)
         outs[tiles] = (pm.x, pm.p, np.asarray(dm))
         del pm
-    for u, v in zip(outs["0"], outs["1"]):
-        assert np.array_equal(u, v)
-    for u, v in zip(outs["2"], outs["3"]):
-        assert np.array_equal(u, v)
-    tol = 1e-12 if prec == "fp64" else 1e-5
-    assert pos_err(outs["0"][0], outs["2"][0], 1.0) <= tol
-    assert rel(outs["0"][1], outs["2"][1]) <= tol
-    assert rel(outs["0"][2] + 1.0, outs["2"][2] + 1.0) <= tol
+    for t in ("1", "2", "3"):
+        for u, v in zip(outs["0"], outs[t]):
+            assert np.array_equal(u, v), t
