@@ -8,7 +8,7 @@
 // pass).  This relies on Np == N and on each particle's base cell being its site or the site just
 // below it on every axis ("near").  A particle outside that range ("far") is recorded by the
 // workgroup that owns its site, in that workgroup's own segment of `far` (no global counter), and is
-// added afterwards by k_deposit_far_seg with global atomics.
+// added afterwards by k_deposit_far_seg with global atomics (several workgroups per segment).
 //
 // Each corner weight ((wx*wy)*wz, the reference's order) is rounded to the fixed-point grid on its
 // own, so every path that deposits a particle adds the same integers; integer accumulation
@@ -176,51 +176,17 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
     }
 }
 
-// Exclusive scan of the per-column far counts (one workgroup): offs[b] = sum of farcnt[0..b),
-// offs[nseg] = total.  Sequential per thread over a chunk, then a scan of the thread sums.
-__global__ __launch_bounds__(kThreads) void k_scan_counts(const unsigned int* __restrict__ farcnt, int nseg,
-                                                          unsigned int* __restrict__ offs) {
-    __shared__ unsigned int part[kThreads];
-    const int chunk = (nseg + kThreads - 1) / kThreads;
-    const int b0 = threadIdx.x * chunk, b1 = min(nseg, b0 + chunk);
-    unsigned int s = 0;
-    for (int b = b0; b < b1; ++b) s += farcnt[b];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int run = 0;
-        for (int t = 0; t < kThreads; ++t) {
-            const unsigned int v = part[t];
-            part[t] = run;
-            run += v;
-        }
-        offs[nseg] = run;
-    }
-    __syncthreads();
-    unsigned int run = part[threadIdx.x];
-    for (int b = b0; b < b1; ++b) {
-        offs[b] = run;
-        run += farcnt[b];
-    }
-}
-
-// Far particles of every column, added with global atomics; threads stride over all of them
-// (segment of far particle t found by binary search in offs), so a column holding most of the far
-// particles (the collapsing core) does not serialise on one workgroup.
+// Far particles, added with global atomics: workgroup (b, y) takes every gridDim.y-th block of
+// kThreads entries of column b's list (a list holds at most the column's own sites).
+constexpr int kFarSub = 4;
 template <class Src, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned long long* __restrict__ mesh,
                                                               const int* __restrict__ far,
-                                                              const unsigned int* __restrict__ offs, int nseg, int seg,
-                                                              int N) {
-    const unsigned int total = offs[nseg];
-    for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < total; t += gridDim.x * kThreads) {
-        int lo_b = 0, hi_b = nseg - 1;  // last segment b with offs[b] <= t
-        while (lo_b < hi_b) {
-            const int mid = (lo_b + hi_b + 1) >> 1;
-            if (offs[mid] <= t) lo_b = mid;
-            else hi_b = mid - 1;
-        }
-        const long long i = far[(size_t)lo_b * seg + (t - offs[lo_b])];
+                                                              const unsigned int* __restrict__ farcnt, int seg, int N) {
+    const unsigned int cnt = farcnt[blockIdx.x];
+    const int* list = far + (size_t)blockIdx.x * seg;
+    for (unsigned int t = blockIdx.y * kThreads + threadIdx.x; t < cnt; t += gridDim.y * kThreads) {
+        const long long i = list[t];
         int lo[3], hi[3];
         typename Src::W w0[3], w1[3];
         src.axes(src.load(i), lo, hi, w0, w1);

@@ -51,7 +51,9 @@ template <typename R, int N, int V> struct Plan {
     static constexpr int PZ = pow2_dividing(pow2_floor(cmin(2 * EB / N, N * N)), N * N);  // r2c rows
     static constexpr int PY = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // y columns
     static constexpr int PX = pow2_dividing(pow2_floor(cmin(EB / N, N / 2)), N / 2);      // x columns
-    static constexpr int PC = pow2_dividing(pow2_floor(cmin(((V & 1) ? EB / 2 : EB) / N, N * N)), N * N);  // c2r rows
+    // c2r rows: NCOMP components of PC rows share one tile of NCOMP*PC pencils of N/2 points
+    template <int NCOMP>
+    static constexpr int pc() { return pow2_dividing(pow2_floor(cmin(EB / ((N / 2) * NCOMP), N * N)), N * N); }
     static constexpr int NBUF = INPLACE ? 1 : 2;
 };
 
@@ -464,40 +466,39 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
 // Half spectra along kz -> real rows, NCOMP components written interleaved out[cell*NCOMP + comp],
 // scaled by `scale` (1/N^3, numpy's backward normalisation).  numpy's c2r uses only the real parts
 // of the kz = 0 and kz = N/2 bins (numpy/fft/_pocketfft.py irfft), and so does this kernel.
+// The NCOMP components of P rows are transformed together as NCOMP*P pencils of one LDS tile
+// (pencil comp*P + c), and the interleaved rows are written straight from that tile.
 template <typename R, int N, int NCOMP, int V>
 __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ gspec, const cplx<R>* __restrict__ gnyq,
                                                     size_t gcs, size_t gcn, R* __restrict__ out,
                                                     const cplx<R>* __restrict__ tw, R scale) {
-    constexpr int NC = N / 2, P = Plan<R, N, V>::PC, PP = P + 1;
+    constexpr int NC = N / 2, P = Plan<R, N, V>::template pc<NCOMP>(), NP = NCOMP * P, PP = NP + 1;
     __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
-    __shared__ R O[P * N * NCOMP];
     const size_t row0 = (size_t)blockIdx.x * P;
-    for (int comp = 0; comp < NCOMP; ++comp) {
-        const cplx<R>* sp = gspec + comp * gcs + row0 * NC;
-        const cplx<R>* ny = gnyq + comp * gcn + row0;
-        for (int t = threadIdx.x; t < P * NC; t += kThreads) {
-            const int c = t / NC, k = t - c * NC;
-            cplx<R> xk = sp[(size_t)c * NC + k];
-            cplx<R> xm = (k == 0) ? ny[c] : sp[(size_t)c * NC + NC - k];
-            if (k == 0) { xk.im = (R)0; xm.im = (R)0; }
-            const R er = xk.re + xm.re, ei = xk.im - xm.im;
-            const R dr = xk.re - xm.re, di = xk.im + xm.im;
-            const cplx<R> w = tw[k];  // exp(-2 pi i k/N); the odd part takes exp(+2 pi i k/N)
-            const R orr = dr * w.re + di * w.im, oi = di * w.re - dr * w.im;
-            A[k * PP + c] = cplx<R>{er - oi, ei + orr};
-        }
-        __syncthreads();
-        const cplx<R>* z = lds_fft<R, NC, P, true, 2, radix_l2<R, PK_C2R>(), Plan<R, N, V>::INPLACE>(A, B, tw);
-        for (int t = threadIdx.x; t < P * NC; t += kThreads) {
-            const int c = t / NC, m = t - c * NC;
-            const cplx<R> v = z[m * PP + c];
-            O[(c * N + 2 * m) * NCOMP + comp] = v.re * scale;
-            O[(c * N + 2 * m + 1) * NCOMP + comp] = v.im * scale;
-        }
-        __syncthreads();
+    for (int t = threadIdx.x; t < NP * NC; t += kThreads) {
+        const int q = t / NC, k = t - q * NC;  // q = comp*P + c
+        const int comp = q / P, c = q - comp * P;
+        const cplx<R>* sp = gspec + comp * gcs + (row0 + c) * NC;
+        cplx<R> xk = sp[k];
+        cplx<R> xm = (k == 0) ? gnyq[comp * gcn + row0 + c] : sp[NC - k];
+        if (k == 0) { xk.im = (R)0; xm.im = (R)0; }
+        const R er = xk.re + xm.re, ei = xk.im - xm.im;
+        const R dr = xk.re - xm.re, di = xk.im + xm.im;
+        const cplx<R> w = tw[k];  // exp(-2 pi i k/N); the odd part takes exp(+2 pi i k/N)
+        const R orr = dr * w.re + di * w.im, oi = di * w.re - dr * w.im;
+        A[k * PP + q] = cplx<R>{er - oi, ei + orr};
     }
+    __syncthreads();
+    const cplx<R>* z = lds_fft<R, NC, NP, true, 2, radix_l2<R, PK_C2R>(), Plan<R, N, V>::INPLACE>(A, B, tw);
+    // output element e of row c is component e % NCOMP of cell e / NCOMP; cells 2m and 2m+1 are the
+    // real and imaginary parts of point m.  Consecutive threads write consecutive elements.
     R* dst = out + row0 * N * NCOMP;
-    for (int t = threadIdx.x; t < P * N * NCOMP; t += kThreads) dst[t] = O[t];
+    for (int t = threadIdx.x; t < P * N * NCOMP; t += kThreads) {
+        const int c = t / (N * NCOMP), e = t - c * (N * NCOMP);
+        const int cell = e / NCOMP, comp = e - cell * NCOMP;
+        const cplx<R> v = z[(cell >> 1) * PP + comp * P + c];
+        dst[t] = ((cell & 1) ? v.im : v.re) * scale;
+    }
 }
 
 }  // namespace pm3d

@@ -274,14 +274,16 @@ struct Fft {
 
     template <int V>
     static void c2r(Handle* h, cplx<R>* base, int ncomp, void* out) {
-        const unsigned g = (unsigned)((size_t)N * N / Plan<R, N, V>::PC);
         const R scale = (R)(1.0 / ((double)N * N * N));
         const cplx<R>* tw = (const cplx<R>*)h->tw;
         Timed t(h, KC_C2R_Z);
-        if (ncomp == 3)
+        if (ncomp == 3) {
+            const unsigned g = (unsigned)((size_t)N * N / Plan<R, N, V>::template pc<3>());
             k_c2r_z<R, N, 3, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
-        else
+        } else {
+            const unsigned g = (unsigned)((size_t)N * N / Plan<R, N, V>::template pc<1>());
             k_c2r_z<R, N, 1, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+        }
     }
 
     // r2c z + forward y on h->spec.  fixed: input is the deposit mesh.
@@ -450,9 +452,10 @@ void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh) {
             src, mesh, h->far, h->farcnt, XMap{1, 1});
     }
     Timed t(h, KC_DEPOSIT_FAR);
-    unsigned int* offs = h->farcnt + S::NBLK;
-    k_scan_counts<<<1, kThreads, 0, h->st>>>(h->farcnt, S::NBLK, offs);
-    k_deposit_far_seg<Src, WIDE><<<512, kThreads, 0, h->st>>>(src, mesh, h->far, offs, S::NBLK, S::BX * S::BY * N, N);
+    // a column lists at most its own BX*BY*N sites: kFarSub workgroups per column bound the work of
+    // the fullest one
+    k_deposit_far_seg<Src, WIDE><<<dim3(S::NBLK, kFarSub), kThreads, 0, h->st>>>(src, mesh, h->far, h->farcnt,
+                                                                              S::BX * S::BY * N, N);
 }
 
 template <bool WIDE, class Src>
@@ -810,7 +813,7 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         (rc = dalloc_bytes(h, &h->gspec, 3 * specN * 2 * rb)) || (rc = dalloc_bytes(h, &h->grad, 3 * h->M * rb)) ||
         (rc = dalloc_bytes(h, &h->tw, (size_t)N * 2 * rb)) || (rc = dalloc(h, &h->dscal, 8 + kRing)) ||
         (rc = dalloc(h, &h->bmax, h->bmax_len)) || (rc = dalloc(h, &h->far, (size_t)h->n)) ||
-        (rc = dalloc(h, &h->farcnt, 2 * (size_t)N * N + 1)))  // counts and their exclusive scan
+        (rc = dalloc(h, &h->farcnt, (size_t)N * N)))  // far count per column
         return bail(rc);
     // twiddles exp(-2 pi i k/N) from long-double cos/sin
     std::vector<double> tw(2 * N);
@@ -827,7 +830,7 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         hipMemset(mixed ? (void*)h->accf : (void*)h->acc, 0, n3 * rb) != hipSuccess ||
         hipMemset(h->mesh, 0, (h->wide ? 16 : 8) * (size_t)h->M) != hipSuccess ||
         hipMemset(h->spec, 0, specN * 2 * rb) != hipSuccess || hipMemset(h->gspec, 0, 3 * specN * 2 * rb) != hipSuccess ||
-        hipMemset(h->farcnt, 0, (2 * (size_t)N * N + 1) * sizeof(unsigned int)) != hipSuccess) {
+        hipMemset(h->farcnt, 0, (size_t)N * N * sizeof(unsigned int)) != hipSuccess) {
         h->err = "initial copies failed";
         return bail(PM3D_EHIP);
     }
