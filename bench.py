@@ -49,8 +49,11 @@ def algorithmic_bytes(N: int, n: int, prec: str) -> dict:
         "y_fwd": 2 * c * Mh,
         "x_poisson": c * Mh + 3 * c * Mh,              # read 1, write 3 spectra
         "y_inv": 2 * 3 * c * Mh,
-        "c2r_z": 3 * c * Mh + 3 * r * M,               # read 3 spectra; write interleaved gradient
+        "c2r_z": 3 * c * Mh + 3 * r * M,               # read 3 spectra; write the gradient meshes
         "gather": 24 * n + 3 * r * M + 3 * r * n,      # read x (24 B both modes), gradient; write acc
+        # z and y fused per x plane (N <= 128 FP64, N <= 256 FP32)
+        "fwd_zy": q * M + c * Mh,                      # read mesh; write spectrum
+        "inv_yz": 3 * c * Mh + 3 * r * M,              # read 3 spectra; write the gradient meshes
     }
 
 

@@ -20,20 +20,50 @@
 
 namespace pm3d {
 
-// Particle sources: raw loads of particle i (issued in batches, so that several sites' loads are in
-// flight per thread), the CIC base/upper cells with the per-axis weights, and the fixed-point value
-// of one corner weight (wx*wy)*wz, evaluated in the reference's order (pm3d.py:85-88).
+// Load of base[byte_offset / sizeof(T)] with a 32-bit byte offset (arrays under 4 GiB).
+template <class T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned byte_offset) {
+    return *(const T*)((const char*)base + byte_offset);
+}
+
+// Particle sources.  `load` issues the raw loads of site i (32-bit index off per-axis base pointers,
+// so every load is one instruction); `near_axis` is the column deposit's fast CIC of one axis, valid
+// only when it returns true (base cell = the site or the site below it); `axes` is the general CIC
+// of the far pass; `quant*` round one corner weight (wx*wy)*wz (the reference's order,
+// pm3d.py:85-88) to the fixed-point grid of the mesh.
 struct SrcF64 {
-    const double* x;
-    long long n;
-    double dx, qs;
-    long long one;
+    const double* x0;
+    const double* x1;
+    const double* x2;
+    double dx, inv_dx;
     int N;
     using W = double;
     struct Raw {
         double v[3];
     };
-    __device__ __forceinline__ Raw load(long long i) const { return Raw{{x[i], x[n + i], x[2 * n + i]}}; }
+    __device__ __forceinline__ Raw load(unsigned i) const {
+        const unsigned b = i * 8u;  // byte offset < 2^32: one load with a scalar base and a 32-bit offset
+        return Raw{{ld_off(x0, b), ld_off(x1, b), ld_off(x2, b)}};
+    }
+    __device__ __forceinline__ Raw load(long long i) const { return load((unsigned)i); }
+    // g = X/dx - 0.5, lo = floor(g) mod N, f = g - floor(g) (pm3d.py:76-79).  The quotient is the
+    // Markstein-corrected product of div_dx without its range test: for |X| >= 1e290 the result is
+    // huge or NaN and the particle is sent to the far pass, and for |X| <= 1e-290 any quotient
+    // within a few ulps gives g == -0.5 exactly.  `ob` is the base cell's offset from site s (0 or -1).
+    __device__ __forceinline__ bool near_axis(const Raw& r, int d, int s, int& ob, int& lo, double& f) const {
+        const double X = r.v[d];
+        const double q0 = X * inv_dx;
+        const double q = fma(fma(-q0, dx, X), inv_dx, q0);
+        const double g = q - 0.5;
+        const double fl = floor(g);
+        f = g - fl;
+        const bool inr = fl >= -1.0 && fl <= (double)(N - 1);
+        const int l = (int)(inr ? fl : 0.0);
+        lo = l < 0 ? l + N : l;
+        const int o = lo - s;
+        ob = o == 0 ? 0 : -1;
+        return inr && (o == 0 || o == -1 || o == N - 1);
+    }
     __device__ __forceinline__ void axes(const Raw& r, int lo[3], int hi[3], double w0[3], double w1[3]) const {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -43,21 +73,45 @@ struct SrcF64 {
             w1[d] = f;
         }
     }
+    // weight scale folded into wx before the products (1 here: q52 scales inside its FMA)
+    static constexpr double kScale = 1.0;
     __device__ __forceinline__ unsigned long long quant(double w) const { return q52(w); }
+    // q52 as its two 32-bit halves: 0x4330000000000000 has zero low bits, so no borrow crosses
+    __device__ __forceinline__ void quant2(double w, unsigned& lo32, unsigned& hi32) const {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fma(w, 0x1p52, 0x1p52));
+        lo32 = (unsigned)b;
+        hi32 = (unsigned)(b >> 32) - 0x43300000u;
+    }
 };
 
 struct SrcMx {
-    const int* cell;
-    const float* off;
-    long long n;
+    const int* c0;
+    const int* c1;
+    const int* c2;
+    const float* f0;
+    const float* f1;
+    const float* f2;
     int N;
     using W = float;
     struct Raw {
         int c[3];
         float f[3];
     };
-    __device__ __forceinline__ Raw load(long long i) const {
-        return Raw{{cell[i], cell[n + i], cell[2 * n + i]}, {off[i], off[n + i], off[2 * n + i]}};
+    __device__ __forceinline__ Raw load(unsigned i) const {
+        const unsigned b = i * 4u;
+        return Raw{{ld_off(c0, b), ld_off(c1, b), ld_off(c2, b)}, {ld_off(f0, b), ld_off(f1, b), ld_off(f2, b)}};
+    }
+    __device__ __forceinline__ Raw load(long long i) const { return load((unsigned)i); }
+    // the state keeps the cell in [0, N); a cell outside that range is sent to the far pass, so the
+    // column's LDS index stays in bounds whatever the state holds
+    __device__ __forceinline__ bool near_axis(const Raw& r, int d, int s, int& ob, int& lo, float& f) const {
+        lo = r.c[d];
+        f = r.f[d];
+        const int o = lo - s;
+        ob = o == 0 ? 0 : -1;
+        const bool inr = (unsigned)lo < (unsigned)N;
+        lo = inr ? lo : 0;
+        return inr && (o == 0 || o == -1 || o == N - 1);
     }
     __device__ __forceinline__ void axes(const Raw& r, int lo[3], int hi[3], float w0[3], float w1[3]) const {
 #pragma unroll
@@ -68,22 +122,29 @@ struct SrcMx {
             w1[d] = r.f[d];
         }
     }
+    // 2^31 folded into wx: (2^31 wx * wy) * wz == 2^31 ((wx * wy) * wz) exactly (a power-of-two scale
+    // commutes with rounding; an underflowed product rounds to 0 either way), so rint of the scaled
+    // product is q31 of the corner weight
+    static constexpr float kScale = 2147483648.0f;
     __device__ __forceinline__ unsigned long long quant(float w) const { return q31(w); }
+    __device__ __forceinline__ unsigned quant_scaled(float ws) const { return (unsigned)rintf(ws); }
 };
 
-// Displacement of the base cell from the site along one axis (minimum image).
-__device__ __forceinline__ int site_offset(int lo, int site, int N) {
-    int dd = lo - site;
-    if (dd > N / 2) dd -= N;
-    if (dd < -N / 2) dd += N;
-    return dd;
-}
-
+// Column deposit.  LDS holds the column as BX*BY rows of N+1 cells: entry N of a row stands for
+// z = 0 (the upper z corner of a particle whose base cell is N-1) and is folded into entry 0 before
+// the store, so the upper z corner is always the next entry.  Threads walk the (BX+2) x (BY+2) x N
+// sites of the column and its ring; when 64 | N a wave covers 64 sites of one row, so the row, the
+// site coordinates and the ownership test are wave-uniform.  A corner plane outside the column gets
+// weight 0 and an in-column address, so every lane runs the same instructions; an (x, y) corner pair
+// whose weight is 0 in every lane of the wave (a ring row's far plane, a zero fraction) is skipped.
 template <class Src, int BX, int BY, int N, bool WIDE, int NT>
 __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long* __restrict__ mesh,
                                                     int* __restrict__ far, unsigned int* __restrict__ farcnt, XMap xm) {
-    constexpr int NCELL = BX * BY * N;
+    using W = typename Src::W;
+    constexpr int NZ = N + 1;
+    constexpr int NCELL = BX * BY * NZ;
     constexpr int EX = BX + 2, EY = BY + 2, NSITE = EX * EY * N;
+    constexpr bool ROWS = N % 64 == 0;
     // WIDE: the 128-bit cell is accumulated as two 64-bit sums of the low and high 32-bit halves of
     // each contribution (no return value, no carry test per add) and recombined exactly at the end
     __shared__ unsigned long long acc[NCELL];
@@ -99,79 +160,104 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
     const int col = (int)xcd_map(blockIdx.x, gridDim.x, xm);  // column id, also this column's far segment
     const int x0 = (col / NBY) * BX, y0 = (col % NBY) * BY;
     int* myfar = far + (size_t)col * (BX * BY * N);
+    const int lane = threadIdx.x & 63;
+    const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     constexpr int U = 4;  // sites whose loads are issued together
-    for (int t0 = threadIdx.x; t0 < NSITE; t0 += NT * U) {
+    for (int t0 = 0; t0 < NSITE; t0 += NT * U) {
         typename Src::Raw raw[U];
+        int exs[U], eys[U], szs[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int t = t0 + u * NT;
-            if (t < NSITE) {
-                const int sz = t % N, ey = (t / N) % EY, ex = t / (N * EY);
-                const int sx = wrapN(x0 + ex - 1, N), sy = wrapN(y0 + ey - 1, N);
-                raw[u] = src.load(((long long)sx * N + sy) * N + sz);
+            const int tw = t0 + u * NT + wbase;  // wave-uniform
+            const int t = tw + lane;
+            int row, sz;
+            if (ROWS) {
+                row = tw / N;
+                sz = tw % N + lane;
+            } else {
+                row = t / N;
+                sz = t % N;
+            }
+            exs[u] = row / EY;
+            eys[u] = row % EY;
+            szs[u] = sz;
+            if ((ROWS ? tw : t) < NSITE) {
+                const int sx = wrapN(x0 + exs[u] - 1, N), sy = wrapN(y0 + eys[u] - 1, N);
+                raw[u] = src.load((unsigned)((sx * N + sy) * N + sz));
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-        const int t = t0 + u * NT;
-        if (t >= NSITE) break;
-        const int sz = t % N, ey = (t / N) % EY, ex = t / (N * EY);
-        const int sx = wrapN(x0 + ex - 1, N), sy = wrapN(y0 + ey - 1, N);
-        const long long i = ((long long)sx * N + sy) * N + sz;
-        int lo[3], hi[3];
-        typename Src::W w0[3], w1[3];
-        src.axes(raw[u], lo, hi, w0, w1);
-        const int ox = site_offset(lo[0], sx, N), oy = site_offset(lo[1], sy, N), oz = site_offset(lo[2], sz, N);
-        const bool is_far = ox < -1 || ox > 0 || oy < -1 || oy > 0 || oz < -1 || oz > 0;
-        const bool own = ex >= 1 && ex <= BX && ey >= 1 && ey <= BY;
-        const bool app = is_far && own;
-        const unsigned long long bal = __ballot(app);
-        if (bal) {  // one LDS atomic per wave, slots by prefix popcount
-            const int lane = threadIdx.x & 63;
-            const int leader = __ffsll((long long)bal) - 1;
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(&nfar, (unsigned int)__popcll(bal));
-            base = __shfl(base, leader, 64);
-            if (app) myfar[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)i;
-        }
-        if (is_far) continue;
-        // column-local x/y of the two stencil planes; a plane outside the column maps to >= BX/BY
-        const int rx[2] = {wrapN(lo[0] - x0, N), wrapN(hi[0] - x0, N)};
-        const int ry[2] = {wrapN(lo[1] - y0, N), wrapN(hi[1] - y0, N)};
+            const int tw = t0 + u * NT + wbase;
+            if ((ROWS ? tw : tw + lane) >= NSITE) break;
+            const int ex = exs[u], ey = eys[u], sz = szs[u];
+            const int sx = wrapN(x0 + ex - 1, N), sy = wrapN(y0 + ey - 1, N);
+            int ob[3], lo[3];
+            W f[3];
+            const bool nx = src.near_axis(raw[u], 0, sx, ob[0], lo[0], f[0]);
+            const bool ny = src.near_axis(raw[u], 1, sy, ob[1], lo[1], f[1]);
+            const bool nz = src.near_axis(raw[u], 2, sz, ob[2], lo[2], f[2]);
+            const bool is_near = nx && ny && nz;
+            const bool own = ex >= 1 && ex <= BX && ey >= 1 && ey <= BY;
+            const bool app = !is_near && own;
+            const unsigned long long bal = __ballot(app);
+            if (bal) {  // one LDS atomic per wave, slots by prefix popcount
+                const int leader = __ffsll((long long)bal) - 1;
+                unsigned int base = 0;
+                if (lane == leader) base = atomicAdd(&nfar, (unsigned int)__popcll(bal));
+                base = __shfl(base, leader, 64);
+                if (app) myfar[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)((sx * N + sy) * N + sz);
+            }
+            // column-local planes of the base cell: ex-1+ob in [-2, BX]
+            const int lx = ex - 1 + ob[0], ly = ey - 1 + ob[1];
+            const bool vx0 = is_near && (unsigned)lx < (unsigned)BX, vx1 = is_near && (unsigned)(lx + 1) < (unsigned)BX;
+            const bool vy0 = (unsigned)ly < (unsigned)BY, vy1 = (unsigned)(ly + 1) < (unsigned)BY;
+            const W one = W(1);
+            const W wx[2] = {vx0 ? (one - f[0]) * Src::kScale : W(0), vx1 ? f[0] * Src::kScale : W(0)};
+            const W wy[2] = {vy0 ? one - f[1] : W(0), vy1 ? f[1] : W(0)};
+            const W wz[2] = {one - f[2], f[2]};
+            const int px[2] = {vx0 ? lx : 0, vx1 ? lx + 1 : 0};
+            const int py[2] = {vy0 ? ly : 0, vy1 ? ly + 1 : 0};
 #pragma unroll
-        for (int cxy = 0; cxy < 4; ++cxy) {
-            const int cx = cxy >> 1, cy = cxy & 1;
-            if (rx[cx] < BX && ry[cy] < BY) {  // only the corners that land in this column
-                const int col = (rx[cx] * BY + ry[cy]) * N;
-                const typename Src::W wxy = (cx ? w1[0] : w0[0]) * (cy ? w1[1] : w0[1]);
+            for (int cxy = 0; cxy < 4; ++cxy) {
+                const int cx = cxy >> 1, cy = cxy & 1;
+                const W wxy = wx[cx] * wy[cy];
+                if (__ballot(wxy != W(0)) == 0ull) continue;
+                const int a = (px[cx] * BY + py[cy]) * NZ + lo[2];
 #pragma unroll
                 for (int cz = 0; cz < 2; ++cz) {
-                    const unsigned long long v = src.quant(wxy * (cz ? w1[2] : w0[2]));
-                    if (v != 0ull) {
-                        const int cell = col + (cz ? hi[2] : lo[2]);
-                        if (WIDE) {
-                            atomicAdd(&acc[cell], v & 0xffffffffull);
-                            atomicAdd(&acch[cell], v >> 32);
-                        } else {
-                            atomicAdd(&acc[cell], v);
-                        }
+                    const W w = wxy * wz[cz];
+                    if constexpr (WIDE) {
+                        unsigned l32, h32;
+                        src.quant2(w, l32, h32);
+                        atomicAdd(&acc[a + cz], (unsigned long long)l32);
+                        atomicAdd(&acch[a + cz], (unsigned long long)h32);
+                    } else {
+                        atomicAdd(&acc[a + cz], (unsigned long long)src.quant_scaled(w));
                     }
                 }
             }
         }
-        }
     }
     __syncthreads();
     if (threadIdx.x == 0) farcnt[col] = nfar;
-    for (int t = threadIdx.x; t < NCELL; t += NT) {
-        const int rz = t % N, ry = (t / N) % BY, rx = t / (N * BY);
-        const size_t cell = ((size_t)(x0 + rx) * N + (y0 + ry)) * N + rz;
+    if (threadIdx.x < BX * BY) {  // fold z = N into z = 0
+        const int r = threadIdx.x * NZ;
+        acc[r] += acc[r + N];
+        if (WIDE) acch[r] += acch[r + N];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < BX * BY * N; t += NT) {
+        const int rz = t % N, rxy = t / N;
+        const int ry = rxy % BY, rx = rxy / BY;
+        const unsigned cell = (unsigned)(((x0 + rx) * N + (y0 + ry)) * N + rz);
+        const int l = rxy * NZ + rz;
         if (WIDE) {
-            const unsigned long long L = acc[t], H = acch[t];
+            const unsigned long long L = acc[l], H = acch[l];
             const unsigned long long lo = L + (H << 32);
-            *(ulonglong2*)(mesh + 2 * cell) = make_ulonglong2(lo, (H >> 32) + (lo < L ? 1ull : 0ull));
+            *(ulonglong2*)(mesh + 2 * (size_t)cell) = make_ulonglong2(lo, (H >> 32) + (lo < L ? 1ull : 0ull));
         } else {
-            mesh[cell] = acc[t];
+            mesh[cell] = acc[l];
         }
     }
 }
@@ -202,14 +288,15 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned 
 }
 
 // Column shape per mesh size, cell width and LDS budget: the widest of 8x8, 4x8, 4x4, 2x4
-// columns whose LDS accumulator fits in BUDGET bytes (gfx950 has 160 KB of LDS per CU).
-template <int N, bool WIDE, int BUDGET = 131072> struct ColShape {
+// columns whose LDS accumulator (rows of N+1 cells) fits in BUDGET bytes (gfx950 has 160 KB of LDS
+// per CU).
+template <int N, bool WIDE, int BUDGET = 135168> struct ColShape {
     static constexpr int CELLB = WIDE ? 16 : 8;
-    static constexpr bool fits(int bx, int by) { return bx * by * N * CELLB <= BUDGET; }
+    static constexpr bool fits(int bx, int by) { return bx * by * (N + 1) * CELLB <= BUDGET; }
     static constexpr int BX = fits(8, 8) ? 8 : fits(4, 8) ? 4 : fits(4, 4) ? 4 : 2;
     static constexpr int BY = fits(8, 8) ? 8 : fits(4, 8) ? 8 : fits(4, 4) ? 4 : 4;
     static constexpr int NBLK = (N / BX) * (N / BY);
-    static_assert(BX * BY * N * CELLB <= 163840, "column does not fit in LDS");
+    static_assert(BX * BY * (N + 1) * CELLB <= 163840, "column does not fit in LDS");
 };
 
 }  // namespace pm3d

@@ -327,14 +327,22 @@ __device__ __forceinline__ double khalf(int i, double kval) { return 6.283185307
 // forward FFT along x, then per mode  phik = ((s*dk) * (1/k2)) [* exp((-0.5*k2)*sm2)], phik(0)=0
 // (pm3d.py:105-109; numpy's complex/real division is a multiply by the reciprocal), then for
 // c = x,y,z: -1j*k_c*phik = (k_c*Im, -k_c*Re) (pm3d.py:112) and the inverse FFT along x, written to
-// the three gradient spectra.
+// the three gradient spectra.  If nbmax > 0, the last block also reduces per-block deposit maxima
+// bmax[0..nbmax) into *slot (left there by the plane forward pass, fft_plane.hpp).
 template <typename R, int N, int MODE, int V>
 __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
                                                      cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
                                                      size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
                                                      double kval, double s, double sm2, int smooth_on,
                                                      const DevSched* __restrict__ sch, const DevStep* __restrict__ tab,
-                                                     XMap xm) {
+                                                     const double* __restrict__ bmax, int nbmax,
+                                                     unsigned long long* __restrict__ slot, XMap xm) {
+    if (nbmax > 0 && blockIdx.x == gridDim.x - 1) {
+        double m = 0.0;
+        for (int t = threadIdx.x; t < nbmax; t += kThreads) m = fmax(m, bmax[t]);
+        m = block_max_nonneg(m);
+        if (threadIdx.x == 0) *slot = (unsigned long long)__double_as_longlong(m);
+    }
     if (MODE == 2 && sch) {  // Green's function factor of the scheduled step (last entry past the end)
         const int k = sch->counter < sch->nsteps ? sch->counter : sch->nsteps - 1;
         s = tab[k].s;
@@ -463,14 +471,14 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
 }
 
 // ------------------------------------------------------------------------------------------ z c2r
-// Half spectra along kz -> real rows, NCOMP components written interleaved out[cell*NCOMP + comp],
-// scaled by `scale` (1/N^3, numpy's backward normalisation).  numpy's c2r uses only the real parts
+// Half spectra along kz -> real rows, NCOMP components written component-major
+// out[comp*ocs + cell], scaled by `scale` (1/N^3, numpy's backward normalisation).  numpy's c2r uses only the real parts
 // of the kz = 0 and kz = N/2 bins (numpy/fft/_pocketfft.py irfft), and so does this kernel.
 // The NCOMP components of P rows are transformed together as NCOMP*P pencils of one LDS tile
-// (pencil comp*P + c), and the interleaved rows are written straight from that tile.
+// (pencil comp*P + c), and every row is written straight from that tile.
 template <typename R, int N, int NCOMP, int V>
 __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ gspec, const cplx<R>* __restrict__ gnyq,
-                                                    size_t gcs, size_t gcn, R* __restrict__ out,
+                                                    size_t gcs, size_t gcn, R* __restrict__ out, size_t ocs,
                                                     const cplx<R>* __restrict__ tw, R scale) {
     constexpr int NC = N / 2, P = Plan<R, N, V>::template pc<NCOMP>(), NP = NCOMP * P, PP = NP + 1;
     __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
@@ -490,14 +498,13 @@ __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ 
     }
     __syncthreads();
     const cplx<R>* z = lds_fft<R, NC, NP, true, 2, radix_l2<R, PK_C2R>(), Plan<R, N, V>::INPLACE>(A, B, tw);
-    // output element e of row c is component e % NCOMP of cell e / NCOMP; cells 2m and 2m+1 are the
-    // real and imaginary parts of point m.  Consecutive threads write consecutive elements.
-    R* dst = out + row0 * N * NCOMP;
-    for (int t = threadIdx.x; t < P * N * NCOMP; t += kThreads) {
-        const int c = t / (N * NCOMP), e = t - c * (N * NCOMP);
-        const int cell = e / NCOMP, comp = e - cell * NCOMP;
-        const cplx<R> v = z[(cell >> 1) * PP + comp * P + c];
-        dst[t] = ((cell & 1) ? v.im : v.re) * scale;
+    // cells 2m and 2m+1 of a row are the real and imaginary parts of point m; consecutive threads
+    // write consecutive cells of one row
+    for (int t = threadIdx.x; t < NP * N; t += kThreads) {
+        const int q = t / N, e = t - q * N;  // q = comp*P + c
+        const int comp = q / P, c = q - comp * P;
+        const cplx<R> v = z[(e >> 1) * PP + q];
+        out[comp * ocs + (row0 + c) * N + e] = ((e & 1) ? v.im : v.re) * scale;
     }
 }
 

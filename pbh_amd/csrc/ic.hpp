@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void k_axpy(double* __restrict__ y, const
 }
 
 // Lattice placement: particle i at Q = (i_c + 0.5)*(L/Np) - L/2 (pm3d.py:64-65); the displacement
-// is the CIC gather of the interleaved mesh disp[cell*3 + c] at (Q + L/2) % L (pm3d.py:66), and
+// is the CIC gather of the component-major mesh disp[c*N^3 + cell] at (Q + L/2) % L (pm3d.py:66), and
 // x = ((Q + d) + L/2) % L, p = coef * d (pm3d.py:67-68 with coef = a_i^2 H(a_i)).
 __global__ __launch_bounds__(kThreads) void k_lattice(const double* __restrict__ disp, int N, int Np, double L,
                                                       double coef, double* __restrict__ x, double* __restrict__ p) {
@@ -113,6 +113,7 @@ __global__ __launch_bounds__(kThreads) void k_lattice(const double* __restrict__
     if (i >= n) return;
     const int s[3] = {(int)(i / ((long long)Np * Np)), (int)((i / Np) % Np), (int)(i % Np)};
     const double dq = L / Np, dx = L / N;
+    const size_t M = (size_t)N * N * N;  // displacement mesh disp[d*M + cell]
     double Q[3];
     int lo[3], hi[3];
     double f[3];
@@ -126,9 +127,9 @@ __global__ __launch_bounds__(kThreads) void k_lattice(const double* __restrict__
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-        const double* g = disp + (((size_t)ix * N + iy) * N + iz) * 3;
+        const double* g = disp + (((size_t)ix * N + iy) * N + iz);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) a[d] = a[d] + ((g[d] * wx[c >> 2]) * wy[(c >> 1) & 1]) * wz[c & 1];
+        for (int d = 0; d < 3; ++d) a[d] = a[d] + ((g[d * M] * wx[c >> 2]) * wy[(c >> 1) & 1]) * wz[c & 1];
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
 
 // ------------------------------------------------------------------------------------- gather
 // acc_c = sum over corners ((g_c*wx)*wy)*wz in the reference's order (pm3d.py:91-99,113), from the
-// interleaved gradient mesh grad[cell*3 + c].  KICK: p += (c1*acc)/den (pm3d.py:127), and the
+// component-major gradient mesh grad[c*N^3 + cell].  KICK: p += (c1*acc)/den (pm3d.py:127), and the
 // acceleration is always stored for the opening kick of the next step (pm3d.py:128).
 template <bool KICK>
 __global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__ x, long long n, int N, double dx,
@@ -80,15 +80,16 @@ __global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__
 #pragma unroll
     for (int d = 0; d < 3; ++d) cic_axis_fast(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
     const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
+    const size_t M = (size_t)N * N * N;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-        const double* g = grad + (((size_t)ix * N + iy) * N + iz) * 3;
+        const double* g = grad + (((size_t)ix * N + iy) * N + iz);
         const double u = wx[c >> 2], v = wy[(c >> 1) & 1], w = wz[c & 1];
         a0 = a0 + ((g[0] * u) * v) * w;
-        a1 = a1 + ((g[1] * u) * v) * w;
-        a2 = a2 + ((g[2] * u) * v) * w;
+        a1 = a1 + ((g[M] * u) * v) * w;
+        a2 = a2 + ((g[2 * M] * u) * v) * w;
     }
     acc[i] = a0;
     acc[n + i] = a1;

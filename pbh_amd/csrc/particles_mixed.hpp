@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic_mx(MxPos P, int N, 
     mx_scatter(lo, f, N, mesh);
 }
 
-// CIC gather of the interleaved FP32 gradient mesh into acc (SoA).
+// CIC gather of the component-major FP32 gradient mesh grad[c*N^3 + cell] into acc (SoA).
 __global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const float* __restrict__ grad,
                                                          float* __restrict__ acc, XMap xm) {
     const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
@@ -87,15 +87,16 @@ __global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const f
     P.load(i, lo, f);
     const int hx = lo[0] + 1 == N ? 0 : lo[0] + 1, hy = lo[1] + 1 == N ? 0 : lo[1] + 1, hz = lo[2] + 1 == N ? 0 : lo[2] + 1;
     const float wx[2] = {1.0f - f[0], f[0]}, wy[2] = {1.0f - f[1], f[1]}, wz[2] = {1.0f - f[2], f[2]};
+    const size_t M = (size_t)N * N * N;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int ix = (c >> 2) ? hx : lo[0], iy = ((c >> 1) & 1) ? hy : lo[1], iz = (c & 1) ? hz : lo[2];
-        const float* g = grad + (((size_t)ix * N + iy) * N + iz) * 3;
+        const float* g = grad + (((size_t)ix * N + iy) * N + iz);
         const float w = (wx[c >> 2] * wy[(c >> 1) & 1]) * wz[c & 1];
         a0 = fmaf(g[0], w, a0);
-        a1 = fmaf(g[1], w, a1);
-        a2 = fmaf(g[2], w, a2);
+        a1 = fmaf(g[M], w, a1);
+        a2 = fmaf(g[2 * M], w, a2);
     }
     acc[i] = a0;
     acc[P.n + i] = a1;

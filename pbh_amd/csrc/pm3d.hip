@@ -29,6 +29,7 @@
 #include "particles.hpp"
 #include "particles_mixed.hpp"
 #include "deposit.hpp"
+#include "fft_plane.hpp"
 #include "ic.hpp"
 #include "analyse.hpp"
 
@@ -51,9 +52,12 @@ enum KernelClass {
     KC_C2R_Z,
     KC_GATHER,
     KC_OTHER,
+    KC_FWD_ZY,
+    KC_INV_YZ,
     KC_COUNT
 };
-const char* kKernelNames = "kick_drift,fill,deposit,deposit_far,r2c_z,y_fwd,x_poisson,y_inv,c2r_z,gather,other";
+const char* kKernelNames =
+    "kick_drift,fill,deposit,deposit_far,r2c_z,y_fwd,x_poisson,y_inv,c2r_z,gather,other,fwd_zy,inv_yz";
 
 struct Handle {
     pm3d_cfg cfg{};
@@ -73,7 +77,7 @@ struct Handle {
     unsigned long long* mesh = nullptr;
     void* spec = nullptr;   // [N][N][N/2] + nyq [N][N], complex R
     void* gspec = nullptr;  // 3 x the same
-    void* grad = nullptr;   // [M][3] R
+    void* grad = nullptr;   // [3][M] R (component-major)
     void* tw = nullptr;     // exp(-2 pi i k/N), complex R
     // device scalars: [0] nfar, [1] bad count (sticky until pm3d_synchronize), [2..7] unused,
     // [8 + s % kRing] max rho (double bits) of deposit serial s
@@ -105,6 +109,7 @@ struct Handle {
     std::string err;
     size_t bytes = 0;
     int fftv[5] = {};  // FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z)
+    int bmax_pending = 0;  // per-block deposit maxima left in bmax by the plane forward pass
     // analysis sets (pm3d_set_analysis_sets): indices, offsets, centre/moment/partial buffers
     int* an_idx = nullptr;
     long long* an_off = nullptr;
@@ -197,6 +202,36 @@ enum FftPass { FP_R2C = 0, FP_YFWD, FP_X, FP_YINV, FP_C2R, FP_COUNT };
 template <typename R, int N>
 struct Fft {
     static constexpr size_t NH = N / 2, specN = (size_t)N * N * NH, tot = specN + (size_t)N * N;
+    // z and y passes fused per x plane (fft_plane.hpp) when half a plane fits in LDS
+    static constexpr bool PLANE = PlaneTile<R, N>::FITS;
+
+    static void fwd_plane(Handle* h, const void* in, bool fixed) {
+        constexpr int NT = PlaneTile<R, N>::NT;
+        cplx<R>* spec = (cplx<R>*)h->spec;
+        const cplx<R>* tw = (const cplx<R>*)h->tw;
+        const unsigned g = 2 * N;
+        const XMap xm = xmap(g, 2);  // the two workgroups of a plane read the same rows: one XCD
+        Timed t(h, KC_FWD_ZY);
+        if (fixed && h->wide)
+            k_fwd_plane<R, N, true, true><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal, h->qscale,
+                                                              h->nmean, xm);
+        else if (fixed)
+            k_fwd_plane<R, N, true, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal, h->qscale,
+                                                               h->nmean, xm);
+        else
+            k_fwd_plane<R, N, false, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, nullptr, 0.0, 1.0,
+                                                                xm);
+        h->bmax_pending = fixed ? (int)g : 0;
+    }
+
+    static void inv_plane(Handle* h, cplx<R>* base, int ncomp, void* out) {
+        constexpr int NT = PlaneTile<R, N>::NT;
+        const R scale = (R)(1.0 / ((double)N * N * N));
+        const unsigned g = 2 * N * ncomp;
+        Timed t(h, KC_INV_YZ);
+        k_inv_plane<R, N><<<g, NT, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, (size_t)N * N * N,
+                                               (const cplx<R>*)h->tw, scale, xmap(g, 2));
+    }
 
     // consecutive pencil blocks whose P-element row segments share a 128-byte line
     template <int P>
@@ -249,18 +284,22 @@ struct Fft {
         Timed t(h, KC_X_POISSON);
         const DevSched* sch = h->use_sched ? h->sched : nullptr;
         const XMap xm = xmap(g, pencil_share<P>());
+        // the plane forward pass leaves the deposit maximum in per-block pieces: reduced here
+        const int nb = h->bmax_pending;
+        unsigned long long* slot = nb > 0 ? qmax_slot(h) : nullptr;
+        h->bmax_pending = 0;
         if (mode == 0)
             k_x_pass<R, N, 0, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0,
-                                                            0, nullptr, nullptr, xm);
+                                                            0, nullptr, nullptr, h->bmax, nb, slot, xm);
         else if (mode == 1)
             k_x_pass<R, N, 1, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0,
-                                                            0, nullptr, nullptr, xm);
+                                                            0, nullptr, nullptr, h->bmax, nb, slot, xm);
         else if (mode == 2)
             k_x_pass<R, N, 2, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s,
-                                                            sm2, sm2 > 0.0 ? 1 : 0, sch, h->tab, xm);
+                                                            sm2, sm2 > 0.0 ? 1 : 0, sch, h->tab, h->bmax, nb, slot, xm);
         else
             k_x_pass<R, N, 3, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, 0,
-                                                            0, 0, nullptr, nullptr, xm);
+                                                            0, 0, nullptr, nullptr, h->bmax, nb, slot, xm);
     }
 
     template <int V>
@@ -279,15 +318,21 @@ struct Fft {
         Timed t(h, KC_C2R_Z);
         if (ncomp == 3) {
             const unsigned g = (unsigned)((size_t)N * N / Plan<R, N, V>::template pc<3>());
-            k_c2r_z<R, N, 3, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+            k_c2r_z<R, N, 3, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, (size_t)N * N * N, tw,
+                                                           scale);
         } else {
             const unsigned g = (unsigned)((size_t)N * N / Plan<R, N, V>::template pc<1>());
-            k_c2r_z<R, N, 1, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, tw, scale);
+            k_c2r_z<R, N, 1, V><<<g, kThreads, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, (size_t)N * N * N, tw,
+                                                           scale);
         }
     }
 
     // r2c z + forward y on h->spec.  fixed: input is the deposit mesh.
     static void forward(Handle* h, const void* in, bool fixed) {
+        if constexpr (PLANE) {
+            fwd_plane(h, in, fixed);
+            return;
+        }
         PM3D_V4(h->fftv[FP_R2C], r2c, h, in, fixed);
         const int nb = fixed ? (int)((h->fftv[FP_R2C] & 1) ? r2c_grid<1>() : r2c_grid<0>()) : 0;  // PZ depends on bit 0 only
         PM3D_V4(h->fftv[FP_YFWD], yfwd, h, nb);
@@ -297,9 +342,13 @@ struct Fft {
         PM3D_V4(h->fftv[FP_X], x, h, mode, s, sm2);
     }
 
-    // inverse y (ncomp spectra from base) + c2r z into out (ncomp interleaved)
+    // inverse y (ncomp spectra from base) + c2r z into out (component c at out + c*N^3)
     static void inverse(Handle* h, void* base_, int ncomp, void* out) {
         cplx<R>* base = (cplx<R>*)base_;
+        if constexpr (PLANE) {
+            inv_plane(h, base, ncomp, out);
+            return;
+        }
         PM3D_V4(h->fftv[FP_YINV], yinv, h, base, ncomp);
         PM3D_V4(h->fftv[FP_C2R], c2r, h, base, ncomp, out);
     }
@@ -363,6 +412,24 @@ int fft_inverse(Handle* h, void* base, int ncomp, void* out) {
     return h->mixed ? fft_call<float>(h, 2, nullptr, false, 0, 0, 0, base, ncomp, out)
                     : fft_call<double>(h, 2, nullptr, false, 0, 0, 0, base, ncomp, out);
 }
+// true when the z and y passes run fused per x plane (fft_plane.hpp) for this handle's N
+template <typename R>
+bool plane_for(int N) {
+    switch (N) {
+        case 8: return Fft<R, 8>::PLANE;
+        case 16: return Fft<R, 16>::PLANE;
+        case 32: return Fft<R, 32>::PLANE;
+        case 48: return Fft<R, 48>::PLANE;
+        case 64: return Fft<R, 64>::PLANE;
+        case 96: return Fft<R, 96>::PLANE;
+        case 128: return Fft<R, 128>::PLANE;
+        case 256: return Fft<R, 256>::PLANE;
+        case 512: return Fft<R, 512>::PLANE;
+        default: return false;
+    }
+}
+bool fft_plane_used(const Handle* h) { return h->mixed ? plane_for<float>(h->N) : plane_for<double>(h->N); }
+
 int fft_launch_variant(Handle* h, int pass, int v) {
     return h->mixed ? fft_call<float>(h, 3, nullptr, false, pass, 0, 0, nullptr, v, nullptr)
                     : fft_call<double>(h, 3, nullptr, false, pass, 0, 0, nullptr, v, nullptr);
@@ -405,7 +472,9 @@ int tune_fft(Handle* h) {
     }
     int rc = 0;
     std::array<int, FP_COUNT> best{};
+    const bool plane = fft_plane_used(h);
     for (int p = 0; p < FP_COUNT && !rc; ++p) {
+        if (plane && p != FP_X) continue;  // z and y run in the plane kernels (no tile variants)
         float ms[4] = {0.f, 0.f, 0.f, 0.f};
         for (int v = 0; v < 4 && !rc; ++v) {
             if ((rc = fft_launch_variant(h, p, v))) break;
@@ -440,6 +509,14 @@ int ensure_sbuf(Handle* h, size_t bytes) {
     HCHK(h, hipMalloc(&h->sbuf, bytes));
     h->sbuf_bytes = bytes;
     return 0;
+}
+
+// Particle sources of the column deposit: per-axis base pointers of the SoA state.
+SrcF64 src_f64(const Handle* h) {
+    return SrcF64{h->x, h->x + h->n, h->x + 2 * h->n, h->dx, 1.0 / h->dx, h->N};
+}
+SrcMx src_mx(const Handle* h) {
+    return SrcMx{h->cell, h->cell + h->n, h->cell + 2 * h->n, h->off, h->off + h->n, h->off + 2 * h->n, h->N};
 }
 
 // Column deposit (deposit.hpp) for one mesh size.
@@ -488,7 +565,7 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
     int rc = 0;
     if (!h->mixed) {
         if (tiled) {
-            rc = deposit_columns_n<true>(h, SrcF64{h->x, h->n, h->dx, h->qs, h->qone, h->N}, mesh);
+            rc = deposit_columns_n<true>(h, src_f64(h), mesh);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic<true>
@@ -496,7 +573,7 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
         }
     } else {
         if (tiled) {
-            rc = deposit_columns_n<false>(h, SrcMx{h->cell, h->off, h->n, h->N}, mesh);
+            rc = deposit_columns_n<false>(h, src_mx(h), mesh);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, mesh);
@@ -664,7 +741,8 @@ int poisson_impl(Handle* h, const double* delta, double s, double sm2, double* g
     std::vector<R> ho(3 * h->M);
     HCHK(h, hipMemcpyAsync(ho.data(), h->grad, 3 * h->M * sizeof(R), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
-    for (long long i = 0; i < 3 * h->M; ++i) g[i] = (double)ho[i];
+    for (long long i = 0; i < h->M; ++i)  // device component-major -> (N,N,N,3)
+        for (int c = 0; c < 3; ++c) g[3 * i + c] = (double)ho[c * h->M + i];
     return 0;
 }
 
