@@ -1,0 +1,105 @@
+// Box-grouped launches of the scheduled step for batches (gfx950).
+//
+// A batch replay runs one step of every member.  Launched member by member, each of the nine
+// kernels of a step fills only a few CUs for the smaller boxes (a 32^3 deposit is 16 workgroups)
+// and costs its fixed start-up time once per box, and a captured graph runs its per-box branches one
+// kernel at a time.  Here the members of one mesh size share every launch: the grid is the one-box
+// grid times the number of boxes, and a workgroup reads its box's pointers from `boxes[blockIdx.y]`
+// (blockIdx.z for the far-particle pass, whose y dimension is in use) and runs the same kernel body
+// as the one-box launch, so a member's results do not depend on the grouping.
+#pragma once
+#include "common.hpp"
+#include "deposit.hpp"
+#include "fft.hpp"
+#include "fft_plane.hpp"
+#include "particles_mixed.hpp"
+
+namespace pm3d {
+
+// Per-box pointers and constants of one mixed-precision member (device copy, one per member).
+struct MxBox {
+    int* cell;
+    float* off;
+    float* p;
+    float* acc;
+    long long n;
+    int N, Np;
+    unsigned long long* dscal;  // [0] far counter, [1] sticky bad count
+    DevSched* sched;
+    const DevStep* tab;
+    double* jhist;
+    unsigned long long* mesh;
+    int* far;
+    unsigned int* farcnt;
+    cplx<float>* spec;   // half spectrum; its Nyquist plane follows at + N*N*(N/2)
+    cplx<float>* gspec;  // three gradient spectra of N*N*(N/2) + N*N points each
+    float* grad;         // component-major gradient mesh
+    const cplx<float>* tw;
+    double* bmax;                // per-block maxima / minima
+    unsigned long long* qslot;   // deposit maximum slot
+    double kval;                 // 1/(N dx)
+    double cells_per_site;       // N / Np
+    double qscale, nmean;
+};
+
+__device__ __forceinline__ SrcMx box_src(const MxBox& b) {
+    return SrcMx{b.cell, b.cell + b.n, b.cell + 2 * b.n, b.off, b.off + b.n, b.off + 2 * b.n, b.N};
+}
+
+__global__ __launch_bounds__(kThreads) void k_kick_drift_mx_boxes(const MxBox* __restrict__ boxes) {
+    const MxBox& b = boxes[blockIdx.y];
+    kick_drift_mx_body(b.cell, b.off, b.p, b.acc, b.n, b.N, 1, 0.f, 0.f, 0.f, b.dscal + 1, b.sched, b.tab);
+}
+
+template <int BX, int BY, int N>
+__global__ __launch_bounds__(1024) void k_deposit_col_boxes(const MxBox* __restrict__ boxes, XMap xm) {
+    const MxBox& b = boxes[blockIdx.y];
+    deposit_col_body<SrcMx, BX, BY, N, false, 1024>(box_src(b), b.mesh, b.far, b.farcnt, xm);
+}
+
+__global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox* __restrict__ boxes, int seg) {
+    const MxBox& b = boxes[blockIdx.z];
+    deposit_far_seg_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, seg, b.N);
+}
+
+template <int N>
+__global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
+    const MxBox& b = boxes[blockIdx.y];
+    constexpr size_t specN = (size_t)N * N * (N / 2);
+    fwd_plane_body<float, N, true, false>(b.mesh, b.spec, b.spec + specN, b.tw, b.bmax, b.dscal, b.qscale, b.nmean, xm);
+}
+
+// The Poisson x pass with the scheduled Green's function factor; its last block reduces the
+// plane forward pass's 2N per-block deposit maxima into the box's slot.
+template <int N, int V>
+__global__ __launch_bounds__(kThreads) void k_x_poisson_boxes(const MxBox* __restrict__ boxes, XMap xm) {
+    const MxBox& b = boxes[blockIdx.y];
+    constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
+    x_pass_body<float, N, 2, V>(b.spec, b.spec + specN, b.gspec, b.gspec + specN, tot, tot, b.tw, b.kval, 0.0, 0.0, 0,
+                                b.sched, b.tab, b.bmax, 2 * N, b.qslot, xm);
+}
+
+template <int N>
+__global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_inv_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
+    const MxBox& b = boxes[blockIdx.y];
+    constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
+    inv_plane_body<float, N>(b.gspec, b.gspec + specN, tot, tot, b.grad, (size_t)N * N * N, b.tw,
+                             (float)(1.0 / ((double)N * N * N)), xm);
+}
+
+__global__ __launch_bounds__(kThreads) void k_gather3_mx_boxes(const MxBox* __restrict__ boxes, XMap xm) {
+    const MxBox& b = boxes[blockIdx.y];
+    gather3_mx_body(MxPos{b.cell, b.off, b.n}, b.N, b.grad, b.acc, xm);
+}
+
+__global__ __launch_bounds__(kThreads) void k_min_jacobian_mx_boxes(const MxBox* __restrict__ boxes) {
+    const MxBox& b = boxes[blockIdx.y];
+    min_jacobian_mx_body(MxPos{b.cell, b.off, b.n}, b.Np, b.N, b.cells_per_site, b.bmax);
+}
+
+__global__ __launch_bounds__(kThreads) void k_sched_advance_boxes(const MxBox* __restrict__ boxes, int nb) {
+    const MxBox& b = boxes[blockIdx.y];
+    sched_advance_body(b.sched, b.bmax, nb, b.jhist);
+}
+
+}  // namespace pm3d

@@ -138,8 +138,8 @@ struct SrcMx {
 // weight 0 and an in-column address, so every lane runs the same instructions; an (x, y) corner pair
 // whose weight is 0 in every lane of the wave (a ring row's far plane, a zero fraction) is skipped.
 template <class Src, int BX, int BY, int N, bool WIDE, int NT>
-__global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long* __restrict__ mesh,
-                                                    int* __restrict__ far, unsigned int* __restrict__ farcnt, XMap xm) {
+__device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __restrict__ mesh,
+                                                 int* __restrict__ far, unsigned int* __restrict__ farcnt, XMap xm) {
     using W = typename Src::W;
     constexpr int NZ = N + 1;
     constexpr int NCELL = BX * BY * NZ;
@@ -262,13 +262,19 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
     }
 }
 
+template <class Src, int BX, int BY, int N, bool WIDE, int NT>
+__global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long* __restrict__ mesh,
+                                                    int* __restrict__ far, unsigned int* __restrict__ farcnt, XMap xm) {
+    deposit_col_body<Src, BX, BY, N, WIDE, NT>(src, mesh, far, farcnt, xm);
+}
+
 // Far particles, added with global atomics: workgroup (b, y) takes every gridDim.y-th block of
 // kThreads entries of column b's list (a list holds at most the column's own sites).
 constexpr int kFarSub = 4;
 template <class Src, bool WIDE>
-__global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned long long* __restrict__ mesh,
-                                                              const int* __restrict__ far,
-                                                              const unsigned int* __restrict__ farcnt, int seg, int N) {
+__device__ __forceinline__ void deposit_far_seg_body(Src src, unsigned long long* __restrict__ mesh,
+                                                     const int* __restrict__ far,
+                                                     const unsigned int* __restrict__ farcnt, int seg, int N) {
     const unsigned int cnt = farcnt[blockIdx.x];
     const int* list = far + (size_t)blockIdx.x * seg;
     for (unsigned int t = blockIdx.y * kThreads + threadIdx.x; t < cnt; t += gridDim.y * kThreads) {
@@ -285,6 +291,13 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned 
             fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, v);
         }
     }
+}
+
+template <class Src, bool WIDE>
+__global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned long long* __restrict__ mesh,
+                                                              const int* __restrict__ far,
+                                                              const unsigned int* __restrict__ farcnt, int seg, int N) {
+    deposit_far_seg_body<Src, WIDE>(src, mesh, far, farcnt, seg, N);
 }
 
 // Column shape per mesh size, cell width and LDS budget: the widest of 8x8, 4x8, 4x4, 2x4

@@ -330,13 +330,13 @@ __device__ __forceinline__ double khalf(int i, double kval) { return 6.283185307
 // the three gradient spectra.  If nbmax > 0, the last block also reduces per-block deposit maxima
 // bmax[0..nbmax) into *slot (left there by the plane forward pass, fft_plane.hpp).
 template <typename R, int N, int MODE, int V>
-__global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
-                                                     cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
-                                                     size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
-                                                     double kval, double s, double sm2, int smooth_on,
-                                                     const DevSched* __restrict__ sch, const DevStep* __restrict__ tab,
-                                                     const double* __restrict__ bmax, int nbmax,
-                                                     unsigned long long* __restrict__ slot, XMap xm) {
+__device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
+                                            cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
+                                            size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
+                                            double kval, double s, double sm2, int smooth_on,
+                                            const DevSched* __restrict__ sch, const DevStep* __restrict__ tab,
+                                            const double* __restrict__ bmax, int nbmax,
+                                            unsigned long long* __restrict__ slot, XMap xm) {
     if (nbmax > 0 && blockIdx.x == gridDim.x - 1) {
         double m = 0.0;
         for (int t = threadIdx.x; t < nbmax; t += kThreads) m = fmax(m, bmax[t]);
@@ -468,6 +468,17 @@ __global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec,
             __syncthreads();
         }
     }
+}
+
+template <typename R, int N, int MODE, int V>
+__global__ __launch_bounds__(kThreads) void k_x_pass(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
+                                                     cplx<R>* __restrict__ gspec, cplx<R>* __restrict__ gnyq,
+                                                     size_t gcs, size_t gcn, const cplx<R>* __restrict__ tw,
+                                                     double kval, double s, double sm2, int smooth_on,
+                                                     const DevSched* __restrict__ sch, const DevStep* __restrict__ tab,
+                                                     const double* __restrict__ bmax, int nbmax,
+                                                     unsigned long long* __restrict__ slot, XMap xm) {
+    x_pass_body<R, N, MODE, V>(spec, nyq, gspec, gnyq, gcs, gcn, tw, kval, s, sm2, smooth_on, sch, tab, bmax, nbmax, slot, xm);
 }
 
 // ------------------------------------------------------------------------------------------ z c2r

@@ -175,11 +175,11 @@ struct PlaneLoadBatch<R, false, false> {
 // cells), converted to delta = rho/n_mean - 1 on load (pm3d.py:103-104); both workgroups of a plane
 // store the plane's maximum rho to bmax[2x + h] (reduced by the x pass).  Block 0 clears *nfar.
 template <typename R, int N, bool FIXED, bool WIDE>
-__global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_fwd_plane(const void* __restrict__ in, cplx<R>* __restrict__ spec,
-                                                                   cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
-                                                                   double* __restrict__ bmax,
-                                                                   unsigned long long* __restrict__ nfar, double qscale,
-                                                                   double nmean, XMap xm) {
+__device__ __forceinline__ void fwd_plane_body(const void* __restrict__ in, cplx<R>* __restrict__ spec,
+                                               cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
+                                               double* __restrict__ bmax,
+                                               unsigned long long* __restrict__ nfar, double qscale,
+                                               double nmean, XMap xm) {
     using PT = PlaneTile<R, N>;
     using LD = PlaneLoadBatch<R, FIXED, WIDE>;
     constexpr int H = PT::H, S = PT::S, NT = PT::NT;
@@ -266,16 +266,25 @@ __global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_fwd_plane(const void*
     for (int t = threadIdx.x; t < H; t += NT) nyq[(size_t)x * N + 2 * t + h] = T[H * S + t];
 }
 
+template <typename R, int N, bool FIXED, bool WIDE>
+__global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_fwd_plane(const void* __restrict__ in, cplx<R>* __restrict__ spec,
+                                                                     cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
+                                                                     double* __restrict__ bmax,
+                                                                     unsigned long long* __restrict__ nfar, double qscale,
+                                                                     double nmean, XMap xm) {
+    fwd_plane_body<R, N, FIXED, WIDE>(in, spec, nyq, tw, bmax, nfar, qscale, nmean, xm);
+}
+
 // ---------------------------------------------------------------------------- inverse, plane x
 // Workgroup (comp, x, h), logical id (comp*N + x)*2 + h: inverse y then z c2r of component comp's
 // spectrum on plane x, rows y = 2m + h, written to out[comp*ocs + (x*N + y)*N + z] scaled by
 // `scale` (1/N^3, numpy's backward normalisation).  numpy's c2r uses only the real parts of the
 // kz = 0 and kz = N/2 bins (numpy/fft/_pocketfft.py irfft), and so does this kernel.
 template <typename R, int N>
-__global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_inv_plane(const cplx<R>* __restrict__ gspec,
-                                                                   const cplx<R>* __restrict__ gnyq, size_t gcs,
-                                                                   size_t gcn, R* __restrict__ out, size_t ocs,
-                                                                   const cplx<R>* __restrict__ tw, R scale, XMap xm) {
+__device__ __forceinline__ void inv_plane_body(const cplx<R>* __restrict__ gspec,
+                                               const cplx<R>* __restrict__ gnyq, size_t gcs,
+                                               size_t gcn, R* __restrict__ out, size_t ocs,
+                                               const cplx<R>* __restrict__ tw, R scale, XMap xm) {
     using PT = PlaneTile<R, N>;
     constexpr int H = PT::H, S = PT::S, NT = PT::NT;
     __shared__ cplx<R> T[S * S];
@@ -336,6 +345,14 @@ __global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_inv_plane(const cplx<
         const cplx<R> v = T[(z >> 1) * S + m];
         dst[(size_t)(2 * m) * N + z] = ((z & 1) ? v.im : v.re) * scale;
     }
+}
+
+template <typename R, int N>
+__global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_inv_plane(const cplx<R>* __restrict__ gspec,
+                                                                     const cplx<R>* __restrict__ gnyq, size_t gcs,
+                                                                     size_t gcn, R* __restrict__ out, size_t ocs,
+                                                                     const cplx<R>* __restrict__ tw, R scale, XMap xm) {
+    inv_plane_body<R, N>(gspec, gnyq, gcs, gcn, out, ocs, tw, scale, xm);
 }
 
 }  // namespace pm3d

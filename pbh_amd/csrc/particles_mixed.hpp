@@ -78,8 +78,8 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic_mx(MxPos P, int N, 
 }
 
 // CIC gather of the component-major FP32 gradient mesh grad[c*N^3 + cell] into acc (SoA).
-__global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const float* __restrict__ grad,
-                                                         float* __restrict__ acc, XMap xm) {
+__device__ __forceinline__ void gather3_mx_body(MxPos P, int N, const float* __restrict__ grad,
+                                                float* __restrict__ acc, XMap xm) {
     const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
     if (i >= P.n) return;
     int lo[3];
@@ -103,15 +103,20 @@ __global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const f
     acc[2 * P.n + i] = a2;
 }
 
+__global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const float* __restrict__ grad,
+                                                         float* __restrict__ acc, XMap xm) {
+    gather3_mx_body(P, N, grad, acc, xm);
+}
+
 // Pending closing kick (kp = c1p/denp), opening kick (k1 = c1/den1), drift by p*sd cells
 // (sd = da/(den_d*dx)) with the carry of the offset into the cell index, periodic in [0, N).
 // The cell array is written only where it changes.
-__global__ __launch_bounds__(kThreads) void k_kick_drift_mx(int* __restrict__ cell, float* __restrict__ off,
-                                                            float* __restrict__ p, const float* __restrict__ acc,
-                                                            long long n, int N, int pend, float kp, float k1, float sd,
-                                                            unsigned long long* __restrict__ bad,
-                                                            const DevSched* __restrict__ sch,
-                                                            const DevStep* __restrict__ tab) {
+__device__ __forceinline__ void kick_drift_mx_body(int* __restrict__ cell, float* __restrict__ off,
+                                                   float* __restrict__ p, const float* __restrict__ acc,
+                                                   long long n, int N, int pend, float kp, float k1, float sd,
+                                                   unsigned long long* __restrict__ bad,
+                                                   const DevSched* __restrict__ sch,
+                                                   const DevStep* __restrict__ tab) {
     if (sch) {  // scalars from the device schedule (see k_kick_drift)
         const int k = sch->counter;
         if (k >= sch->nsteps) return;
@@ -150,6 +155,15 @@ __global__ __launch_bounds__(kThreads) void k_kick_drift_mx(int* __restrict__ ce
     }
     const unsigned long long nb = __ballot(!ok);
     if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, (unsigned long long)__popcll(nb));
+}
+
+__global__ __launch_bounds__(kThreads) void k_kick_drift_mx(int* __restrict__ cell, float* __restrict__ off,
+                                                            float* __restrict__ p, const float* __restrict__ acc,
+                                                            long long n, int N, int pend, float kp, float k1, float sd,
+                                                            unsigned long long* __restrict__ bad,
+                                                            const DevSched* __restrict__ sch,
+                                                            const DevStep* __restrict__ tab) {
+    kick_drift_mx_body(cell, off, p, acc, n, N, pend, kp, k1, sd, bad, sch, tab);
 }
 
 __global__ __launch_bounds__(kThreads) void k_kick_mx(float* __restrict__ p, const float* __restrict__ acc, long long n3,
@@ -203,8 +217,8 @@ __global__ __launch_bounds__(kThreads) void k_f32_soa_to_f64(const float* __rest
 
 // Last kernel of a scheduled step: reduce the Jacobian kernel's per-block minima into
 // jhist[counter] and advance the schedule.
-__global__ __launch_bounds__(kThreads) void k_sched_advance(DevSched* __restrict__ sch, const double* __restrict__ bmin,
-                                                            int nb, double* __restrict__ jhist) {
+__device__ __forceinline__ void sched_advance_body(DevSched* __restrict__ sch, const double* __restrict__ bmin,
+                                                   int nb, double* __restrict__ jhist) {
     double m = 1e300;
     for (int t = threadIdx.x; t < nb; t += kThreads) m = fmin(m, bmin[t]);
     m = block_min(m);
@@ -217,9 +231,14 @@ __global__ __launch_bounds__(kThreads) void k_sched_advance(DevSched* __restrict
     }
 }
 
+__global__ __launch_bounds__(kThreads) void k_sched_advance(DevSched* __restrict__ sch, const double* __restrict__ bmin,
+                                                            int nb, double* __restrict__ jhist) {
+    sched_advance_body(sch, bmin, nb, jhist);
+}
+
 // min det(dx/dq) over the Lagrangian lattice from cell + offset (differences exact in cells).
-__global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, int N, double cells_per_site,
-                                                              double* __restrict__ bmin) {
+__device__ __forceinline__ void min_jacobian_mx_body(MxPos P, int Np, int N, double cells_per_site,
+                                                     double* __restrict__ bmin) {
     const long long n = P.n;
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     double det = 1e300;
@@ -244,6 +263,11 @@ __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, i
     }
     const double m = block_min(det);
     if (threadIdx.x == 0) bmin[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, int N, double cells_per_site,
+                                                              double* __restrict__ bmin) {
+    min_jacobian_mx_body(P, Np, N, cells_per_site, bmin);
 }
 
 }  // namespace pm3d

@@ -30,6 +30,7 @@
 #include "particles_mixed.hpp"
 #include "deposit.hpp"
 #include "fft_plane.hpp"
+#include "batch_kernels.hpp"
 #include "ic.hpp"
 #include "analyse.hpp"
 
@@ -786,8 +787,99 @@ struct Batch {
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> joins;
     std::vector<Handle*> members;
+    std::vector<MxBox*> dbox;  // device descriptor arrays of the box-grouped classes
     std::string err;
 };
+
+// Members that share the box-grouped launches (batch_kernels.hpp): mixed precision, particles on
+// the mesh lattice (column deposit), plane FFT.
+bool box_groupable(const Handle* h) {
+    if (!h->mixed || h->Np != h->N || h->N < 16 || !fft_plane_used(h)) return false;
+    switch (h->N) {
+        case 16: case 32: case 48: case 64: case 96: case 128: case 256: return true;
+        default: return false;
+    }
+}
+
+MxBox box_of(Handle* h) {
+    const size_t specN = (size_t)h->N * h->N * (h->N / 2);
+    MxBox b{};
+    b.cell = h->cell;
+    b.off = h->off;
+    b.p = h->pf;
+    b.acc = h->accf;
+    b.n = h->n;
+    b.N = h->N;
+    b.Np = h->Np;
+    b.dscal = h->dscal;
+    b.sched = h->sched;
+    b.tab = h->tab;
+    b.jhist = h->jhist;
+    b.mesh = h->mesh;
+    b.far = h->far;
+    b.farcnt = h->farcnt;
+    b.spec = (cplx<float>*)h->spec;
+    b.gspec = (cplx<float>*)h->gspec;
+    b.grad = (float*)h->grad;
+    b.tw = (const cplx<float>*)h->tw;
+    b.bmax = h->bmax;
+    b.qslot = qmax_slot(h);
+    b.kval = 1.0 / ((double)h->N * h->dx);
+    b.cells_per_site = (double)h->N / h->Np;
+    b.qscale = h->qscale;
+    b.nmean = h->nmean;
+    (void)specN;
+    return b;
+}
+
+// One scheduled step of nb boxes of mesh size N in nine launches (the kernels and grids of
+// launch_sched_step per box, with the boxes along the grid's y (far pass: z) dimension).
+template <int N>
+void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
+    using S = ColShape<N, false>;
+    const unsigned g = grid_for(h0->n);
+    k_kick_drift_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev);
+    k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, XMap{1, 1});
+    k_deposit_far_seg_boxes<<<dim3(S::NBLK, kFarSub, nb), kThreads, 0, st>>>(dev, S::BX * S::BY * N);
+    k_fwd_plane_boxes<N><<<dim3(2 * N, nb), PlaneTile<float, N>::NT, 0, st>>>(dev, xmap(2 * N, 2));
+    switch (h0->fftv[FP_X]) {
+#define PM3D_XBOX(V)                                                                                    \
+    case V: {                                                                                           \
+        constexpr int P = Plan<float, N, V>::PX;                                                        \
+        const unsigned gx = N * (N / 2 / P) + N / P;                                                    \
+        k_x_poisson_boxes<N, V><<<dim3(gx, nb), kThreads, 0, st>>>(                                     \
+            dev, xmap(gx, Fft<float, N>::template pencil_share<P>()));                                  \
+        break;                                                                                          \
+    }
+        PM3D_XBOX(1)
+        PM3D_XBOX(2)
+        PM3D_XBOX(3)
+        default:
+        PM3D_XBOX(0)
+#undef PM3D_XBOX
+    }
+    k_inv_plane_boxes<N><<<dim3(6 * N, nb), PlaneTile<float, N>::NT, 0, st>>>(dev, xmap(6 * N, 2));
+    k_gather3_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev, gather_map(h0));
+    k_min_jacobian_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev);
+    k_sched_advance_boxes<<<dim3(1, nb), kThreads, 0, st>>>(dev, (int)g);
+}
+
+void launch_boxes(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
+    switch (h0->N) {
+        case 16: launch_boxes_n<16>(h0, dev, nb, st); break;
+        case 32: launch_boxes_n<32>(h0, dev, nb, st); break;
+        case 48: launch_boxes_n<48>(h0, dev, nb, st); break;
+        case 64: launch_boxes_n<64>(h0, dev, nb, st); break;
+        case 96: launch_boxes_n<96>(h0, dev, nb, st); break;
+        case 128: launch_boxes_n<128>(h0, dev, nb, st); break;
+        default: launch_boxes_n<256>(h0, dev, nb, st); break;
+    }
+}
+
+void batch_free_boxes(Batch* b) {
+    for (MxBox* d : b->dbox) hipFree(d);
+    b->dbox.clear();
+}
 
 void batch_clear_graph(Batch* b) {
     if (b->exec) hipGraphExecDestroy(b->exec);
@@ -1498,6 +1590,7 @@ int pm3d_batch_destroy(void* batch) {
     Batch* b = (Batch*)batch;
     hipStreamSynchronize(b->st);
     batch_clear_graph(b);
+    batch_free_boxes(b);
     for (auto e : b->joins) hipEventDestroy(e);
     hipEventDestroy(b->fork);
     hipStreamDestroy(b->st);
@@ -1512,6 +1605,7 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
     Batch* b = (Batch*)batch;
     hipStreamSynchronize(b->st);
     batch_clear_graph(b);
+    batch_free_boxes(b);
     b->members.clear();
     for (int i = 0; i < count; ++i) {
         Handle* h = (Handle*)handles[i];
@@ -1534,14 +1628,46 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
         }
         b->joins.push_back(e);
     }
-    // capture one step of every member as independent branches of one graph
+    // members of one mesh size that can share launches: their descriptors, uploaded before capture
+    std::map<int, std::vector<int>> classes;
+    std::vector<bool> grouped(count, false);
+    for (int i = 0; i < count; ++i)
+        if (box_groupable(b->members[i])) classes[b->members[i]->N].push_back(i);
+    std::vector<std::pair<int, MxBox*>> cls_dev;  // (first member, descriptors)
+    for (auto& kv : classes) {
+        std::vector<MxBox> hb;
+        for (int i : kv.second) {
+            b->members[i]->serial += 1;  // as deposit_state does in a one-box capture: the replays' deposit
+            hb.push_back(box_of(b->members[i]));
+            grouped[i] = true;
+        }
+        MxBox* d = nullptr;
+        if (hipMalloc((void**)&d, hb.size() * sizeof(MxBox)) != hipSuccess) {
+            b->err = "hipMalloc of batch descriptors failed";
+            return PM3D_ENOMEM;
+        }
+        b->dbox.push_back(d);
+        if (hipMemcpy(d, hb.data(), hb.size() * sizeof(MxBox), hipMemcpyHostToDevice) != hipSuccess) {
+            b->err = "copy of batch descriptors failed";
+            return PM3D_EHIP;
+        }
+        cls_dev.push_back({kv.second[0], d});
+    }
+    // capture one step: one set of launches per class on the batch stream, and every other member as
+    // an independent branch on its own stream
     if (hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed) != hipSuccess) {
         b->err = "hipStreamBeginCapture failed";
         return PM3D_EHIP;
     }
     hipError_t e = hipEventRecord(b->fork, b->st);
     int rc = 0;
+    for (size_t c = 0; c < cls_dev.size() && e == hipSuccess; ++c) {
+        Handle* h0 = b->members[cls_dev[c].first];
+        launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), b->st);
+        e = hipGetLastError();
+    }
     for (int i = 0; i < count && e == hipSuccess && rc == 0; ++i) {
+        if (grouped[i]) continue;
         Handle* h = b->members[i];
         const bool timing = h->timing;
         h->timing = false;
