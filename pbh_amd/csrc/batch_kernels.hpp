@@ -79,12 +79,12 @@ __global__ __launch_bounds__(kThreads) void k_x_poisson_boxes(const MxBox* __res
                                 b.sched, b.tab, b.bmax, 2 * N, b.qslot, xm);
 }
 
-template <int N>
-__global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_inv_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
+template <int N, int Q>
+__global__ __launch_bounds__((InvTile<float, N, Q>::NT)) void k_inv_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
     constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
-    inv_plane_body<float, N>(b.gspec, b.gspec + specN, tot, tot, b.grad, (size_t)N * N * N, b.tw,
-                             (float)(1.0 / ((double)N * N * N)), xm);
+    inv_plane_body<float, N, Q>(b.gspec, b.gspec + specN, tot, tot, b.grad, (size_t)N * N * N, b.tw,
+                                (float)(1.0 / ((double)N * N * N)), xm);
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather3_mx_boxes(const MxBox* __restrict__ boxes, XMap xm) {

@@ -17,9 +17,11 @@
 //            followed by an N/2-point inverse y transform and the z c2r of those rows.
 // The workgroup's tile is (N/2+1) x (N/2+1) complex: kz = 0..N/2 (Nyquist included) by N/2 rows.
 // Tile element (a, b) is at a*S + b with S = N/2+1; the z transforms treat rows as pencils
-// (element kz at kz*S + row, pencil-fastest threads), the y transforms treat kz columns as pencils
-// (element row at kz*S + row, element-fastest threads), so LDS accesses of a wave are contiguous
-// runs in both.
+// (element kz at kz*S + row), the y transforms treat kz columns as pencils (element row at
+// kz*S + row).  Consecutive threads take consecutive pencils in both, so a wave's LDS accesses are
+// a contiguous run (z) or a run of stride S (y); S is odd, so neither conflicts on the banks.  (With
+// consecutive butterflies of one pencil per thread instead, the first radix-R pass of the y
+// transform writes at a stride of R elements, 128 bytes, one bank for the whole wave.)
 #pragma once
 #include "common.hpp"
 #include "fft.hpp"
@@ -257,7 +259,7 @@ __device__ __forceinline__ void fwd_plane_body(const void* __restrict__ in, cplx
     }
     __syncthreads();
     // y: H-point transform of every kz column, kz = 0..H (pencil kz, element y')
-    tile_fft<R, H, S, 1, S, false, false, 2, PT::ML2, NT>(T, tw);
+    tile_fft<R, H, S, 1, S, true, false, 2, PT::ML2, NT>(T, tw);
     // ky = 2m + h
     for (int t = threadIdx.x; t < H * H; t += NT) {
         const int kz = t % H, m = t / H;
@@ -276,83 +278,118 @@ __global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_fwd_plane(const void*
 }
 
 // ---------------------------------------------------------------------------- inverse, plane x
-// Workgroup (comp, x, h), logical id (comp*N + x)*2 + h: inverse y then z c2r of component comp's
-// spectrum on plane x, rows y = 2m + h, written to out[comp*ocs + (x*N + y)*N + z] scaled by
-// `scale` (1/N^3, numpy's backward normalisation).  numpy's c2r uses only the real parts of the
-// kz = 0 and kz = N/2 bins (numpy/fft/_pocketfft.py irfft), and so does this kernel.
+// Tile of the inverse plane kernel split Q ways along y: rows y = Q*m + q (m < N/Q) of one
+// component's plane; T[kz*RS + m], kz = 0..N/2, RS = N/Q + 1.  A smaller tile (Q = 4, 8) leaves
+// room for more workgroups per CU, whose load, LDS and store phases then overlap; each workgroup
+// reads the whole component plane, the Q workgroups of a plane run on one XCD (the repeats come
+// from L2).
+template <typename R, int N, int Q>
+struct InvTile {
+    static constexpr int H = N / 2, S = H + 1, RW = N / Q, RS = RW + 1;
+    static constexpr int BYTES = S * RS * (int)sizeof(cplx<R>);
+    static constexpr bool FITS = N >= 2 * Q && BYTES <= 160 * 1024 - 512;
+    static constexpr int ML2 = PlaneTile<R, N>::ML2;
+    // threads: about one per butterfly of the z pass (H/radix per row, RW rows), 256..1024
+    static constexpr int WORK = RW * (H >> ML2);
+    static constexpr int NT = WORK >= 1024 ? 1024 : WORK >= 512 ? 512 : 256;
+};
+
+// a * exp(+2 pi i e/8) for e in 0..7: exact for the quarter turns, twiddle product otherwise
+// (tw[k] = exp(-2 pi i k/N), so exp(+2 pi i e/8) = conj(tw[e*N/8])).
 template <typename R, int N>
+__device__ __forceinline__ cplx<R> turn8(cplx<R> a, int e, const cplx<R>* __restrict__ tw) {
+    switch (e & 7) {
+        case 0: return a;
+        case 2: return cplx<R>{-a.im, a.re};
+        case 4: return cplx<R>{-a.re, -a.im};
+        case 6: return cplx<R>{a.im, -a.re};
+        default: return cmul(a, cconj(tw[(e & 7) * (N / 8)]));
+    }
+}
+
+// Workgroup (comp, x, q), logical id (comp*N + x)*Q + q: inverse y then z c2r of component comp's
+// spectrum on plane x, rows y = Q*m + q, written to out[comp*ocs + (x*N + y)*N + z] scaled by
+// `scale` (1/N^3, numpy's backward normalisation).  The first radix-Q stage of the inverse y
+// transform (decimation in frequency) is done while loading:
+//   B_q[k'] = (sum_j A[k' + j*N/Q] exp(+2 pi i q j/Q)) exp(+2 pi i q k'/N),  k' < N/Q,
+// then an N/Q-point inverse y transform gives rows m.  numpy's c2r uses only the real parts of the
+// kz = 0 and kz = N/2 bins (numpy/fft/_pocketfft.py irfft), and so does this kernel.
+template <typename R, int N, int Q>
 __device__ __forceinline__ void inv_plane_body(const cplx<R>* __restrict__ gspec,
                                                const cplx<R>* __restrict__ gnyq, size_t gcs,
                                                size_t gcn, R* __restrict__ out, size_t ocs,
                                                const cplx<R>* __restrict__ tw, R scale, XMap xm) {
-    using PT = PlaneTile<R, N>;
-    constexpr int H = PT::H, S = PT::S, NT = PT::NT;
-    __shared__ cplx<R> T[S * S];
+    using IT_ = InvTile<R, N, Q>;
+    constexpr int H = IT_::H, S = IT_::S, RW = IT_::RW, RS = IT_::RS, NT = IT_::NT;
+    static_assert(Q == 2 || Q == 4 || Q == 8, "split 2, 4 or 8 ways");
+    __shared__ cplx<R> T[S * RS];
     const unsigned lin = xcd_map(blockIdx.x, gridDim.x, xm);
-    const int comp = (int)(lin / (2 * N)), x = (int)((lin >> 1) % N), h = (int)(lin & 1);
+    const int comp = (int)(lin / (Q * N)), x = (int)((lin / Q) % N), q = (int)(lin % Q);
     const cplx<R>* sp = gspec + comp * gcs + (size_t)x * N * H;
     const cplx<R>* np = gnyq + comp * gcn + (size_t)x * N;
-    // B[k'][kz] = (A[k'] +/- A[k'+H]) (* exp(+2 pi i k'/N) for h = 1) at T[kz*S + k']; each thread
-    // issues the loads of LB elements before it stores any of them
-    constexpr int IT = (H * S + NT - 1) / NT, LB = sizeof(R) == 8 ? 4 : 8;
+    // each thread issues the loads of LB elements (Q spectrum values each) before it stores any
+    constexpr int IT = (RW * S + NT - 1) / NT, LB = (sizeof(R) == 8 ? 8 : 16) / Q > 0 ? (sizeof(R) == 8 ? 8 : 16) / Q : 1;
     for (int i0 = 0; i0 < IT; i0 += LB) {
-        cplx<R> a[LB], b[LB], w[LB];
+        cplx<R> a[LB][Q], w[LB];
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             const int t = threadIdx.x + (i0 + u) * NT;
-            if (t < H * S) {
+            if (i0 + u < IT && t < RW * S) {
                 const int kz = t % S, kp = t / S;
                 const cplx<R>* pa = kz < H ? sp + (size_t)kp * H + kz : np + kp;
-                const size_t ob = kz < H ? (size_t)H * H : (size_t)H;
-                a[u] = pa[0];
-                b[u] = pa[ob];
-                w[u] = tw[kp];
+                const size_t ob = kz < H ? (size_t)RW * H : (size_t)RW;
+#pragma unroll
+                for (int jj = 0; jj < Q; ++jj) a[u][jj] = pa[jj * ob];
+                w[u] = tw[(q * kp) % N];
             }
         }
 #pragma unroll
         for (int u = 0; u < LB; ++u) {
             const int t = threadIdx.x + (i0 + u) * NT;
-            if (t < H * S) {
+            if (i0 + u < IT && t < RW * S) {
                 const int kz = t % S, kp = t / S;
-                T[kz * S + kp] = h ? cmul(csub(a[u], b[u]), cconj(w[u])) : cadd(a[u], b[u]);
+                cplx<R> v = a[u][0];
+#pragma unroll
+                for (int jj = 1; jj < Q; ++jj) v = cadd(v, turn8<R, N>(a[u][jj], (q * jj * (8 / Q)) & 7, tw));
+                T[kz * RS + kp] = q ? cmul(v, cconj(w[u])) : v;
             }
         }
     }
     __syncthreads();
-    // inverse y: H-point transform of every kz column (pencil kz, element k')
-    tile_fft<R, H, S, 1, S, false, true, 2, PT::ML2, NT>(T, tw);
+    // inverse y: RW-point transform of every kz column (pencil kz, element k')
+    tile_fft<R, RW, S, 1, RS, true, true, Q, IT_::ML2, NT>(T, tw);
     // c2r input points of every row m (pencil m, element k): point k from bins k and H-k (bin 0
     // pairs with the Nyquist bin; the imaginary parts of both are dropped)
-    for (int t = threadIdx.x; t < (H / 2 + 1) * H; t += NT) {
-        const int m = t % H, k = t / H;
-        cplx<R> xk = T[k * S + m];
-        cplx<R> xm = T[(k == 0 ? H : H - k) * S + m];
+    for (int t = threadIdx.x; t < (H / 2 + 1) * RW; t += NT) {
+        const int m = t % RW, k = t / RW;
+        cplx<R> xk = T[k * RS + m];
+        cplx<R> xm = T[(k == 0 ? H : H - k) * RS + m];
         if (k == 0) {
             xk.im = (R)0;
             xm.im = (R)0;
         }
         const cplx<R> pk = c2r_point(xk, xm, tw[k]);
-        if (k != 0 && k != H - k) T[(H - k) * S + m] = c2r_point(xm, xk, tw[H - k]);
-        T[k * S + m] = pk;
+        if (k != 0 && k != H - k) T[(H - k) * RS + m] = c2r_point(xm, xk, tw[H - k]);
+        T[k * RS + m] = pk;
     }
     __syncthreads();
     // z: H-point inverse transform of every row
-    tile_fft<R, H, H, S, 1, true, true, 2, PT::ML2, NT>(T, tw);
-    // cells 2j and 2j+1 of row y = 2m + h are the real and imaginary parts of point j
-    R* dst = out + comp * ocs + ((size_t)x * N + h) * N;
-    for (int t = threadIdx.x; t < H * N; t += NT) {
+    tile_fft<R, H, RW, RS, 1, true, true, 2, IT_::ML2, NT>(T, tw);
+    // cells 2j and 2j+1 of row y = Q*m + q are the real and imaginary parts of point j
+    R* dst = out + comp * ocs + ((size_t)x * N + q) * N;
+    for (int t = threadIdx.x; t < RW * N; t += NT) {
         const int z = t % N, m = t / N;
-        const cplx<R> v = T[(z >> 1) * S + m];
-        dst[(size_t)(2 * m) * N + z] = ((z & 1) ? v.im : v.re) * scale;
+        const cplx<R> v = T[(z >> 1) * RS + m];
+        dst[(size_t)(Q * m) * N + z] = ((z & 1) ? v.im : v.re) * scale;
     }
 }
 
-template <typename R, int N>
-__global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_inv_plane(const cplx<R>* __restrict__ gspec,
-                                                                     const cplx<R>* __restrict__ gnyq, size_t gcs,
-                                                                     size_t gcn, R* __restrict__ out, size_t ocs,
-                                                                     const cplx<R>* __restrict__ tw, R scale, XMap xm) {
-    inv_plane_body<R, N>(gspec, gnyq, gcs, gcn, out, ocs, tw, scale, xm);
+template <typename R, int N, int Q>
+__global__ __launch_bounds__((InvTile<R, N, Q>::NT)) void k_inv_plane(const cplx<R>* __restrict__ gspec,
+                                                                      const cplx<R>* __restrict__ gnyq, size_t gcs,
+                                                                      size_t gcn, R* __restrict__ out, size_t ocs,
+                                                                      const cplx<R>* __restrict__ tw, R scale, XMap xm) {
+    inv_plane_body<R, N, Q>(gspec, gnyq, gcs, gcn, out, ocs, tw, scale, xm);
 }
 
 }  // namespace pm3d

@@ -225,14 +225,20 @@ struct Fft {
         h->bmax_pending = fixed ? (int)g : 0;
     }
 
-    static void inv_plane(Handle* h, cplx<R>* base, int ncomp, void* out) {
-        constexpr int NT = PlaneTile<R, N>::NT;
+    template <int Q>
+    static void inv_plane_q(Handle* h, cplx<R>* base, int ncomp, void* out) {
+        constexpr int NT = InvTile<R, N, Q>::NT;
         const R scale = (R)(1.0 / ((double)N * N * N));
-        const unsigned g = 2 * N * ncomp;
+        const unsigned g = Q * N * ncomp;
         Timed t(h, KC_INV_YZ);
-        k_inv_plane<R, N><<<g, NT, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, (size_t)N * N * N,
-                                               (const cplx<R>*)h->tw, scale, xmap(g, 2));
+        k_inv_plane<R, N, Q><<<g, NT, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, (size_t)N * N * N,
+                                                  (const cplx<R>*)h->tw, scale, xmap(g, Q));
     }
+    // split 2 ways along y: measured on MI355X against 4 and 8 (smaller tiles, more workgroups per
+    // CU, the plane read 4 or 8 times from L2), 128^3 FP64 / 256^3 FP32 inverse pass: 30.2 / 159 us
+    // (2), 32.7 / 189 us (4), 40.1 / 216 us (8, before the bank-conflict fix of the y phase)
+    static constexpr int kInvQ = 2;
+    static void inv_plane(Handle* h, cplx<R>* base, int ncomp, void* out) { inv_plane_q<kInvQ>(h, base, ncomp, out); }
 
     // consecutive pencil blocks whose P-element row segments share a 128-byte line
     template <int P>
@@ -858,7 +864,10 @@ void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
         PM3D_XBOX(0)
 #undef PM3D_XBOX
     }
-    k_inv_plane_boxes<N><<<dim3(6 * N, nb), PlaneTile<float, N>::NT, 0, st>>>(dev, xmap(6 * N, 2));
+    {
+        constexpr int Q = Fft<float, N>::kInvQ;
+        k_inv_plane_boxes<N, Q><<<dim3(3 * Q * N, nb), InvTile<float, N, Q>::NT, 0, st>>>(dev, xmap(3 * Q * N, Q));
+    }
     k_gather3_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev, gather_map(h0));
     k_min_jacobian_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev);
     k_sched_advance_boxes<<<dim3(1, nb), kThreads, 0, st>>>(dev, (int)g);
