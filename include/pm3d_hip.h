@@ -41,7 +41,8 @@ typedef struct pm3d_cfg {
     double L;           /* periodic box side */
     double smooth;      /* Gaussian force smoothing length in cells, 0 = none (pm3d.py:27,107-108) */
     int32_t precision;  /* PM3D_FP64 or PM3D_MIXED */
-    int32_t frac_bits;  /* fixed-point fraction bits of the deposit; 0 selects 40 */
+    int32_t frac_bits;  /* fixed-point fraction bits of the deposit: 0 (or the mode's own value) selects
+                         * 52 on 128-bit cells (FP64) or 31 on 64-bit cells (mixed); others are rejected */
 } pm3d_cfg;
 
 /* Host scalars of one KDK step, evaluated in Python exactly as pm3d.py:118-127 does
