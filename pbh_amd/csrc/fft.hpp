@@ -13,8 +13,9 @@
 // kz = N/2 Nyquist plane lives in a separate [kx][ky] array `nyq`.  numpy's layout (N,N,N/2+1) is
 // assembled only on the host side of the test entry points.
 //
-// LDS layout of a pencil block: element j of pencil c at buf[j*(P+1) + c]; threads map pencil-fastest
-// in the butterflies so every LDS access of a wave is a contiguous run.
+// LDS layout of a pencil block: element j of pencil c at buf[j*PP + c], PP = lds_pad(P) (P+1, or P+2
+// for FP32 with P = 8); threads map pencil-fastest in the butterflies, so every LDS access of a wave
+// is a run of contiguous pencils.
 #pragma once
 #include "common.hpp"
 
@@ -119,10 +120,18 @@ __device__ __forceinline__ void dft_reg(cplx<R>* v) {
     }
 }
 
+// Row stride of a P-pencil LDS tile buf[j*PP + c].  Threads take consecutive pencils, so a wave
+// reads 64/P rows of P elements.  P + 1 spreads the rows over the banks, except for 2-dword
+// elements (FP32 complex) with P = 8, where a stride of 2*(P+1) = 18 dwords starts the 8 rows of a
+// wave unevenly (about 2e7 bank-conflict cycles per 256^3 x-pass launch on MI355X); P + 2 starts
+// them on every fourth bank, so each bank serves 4 of the wave's 128 dwords.
+template <typename R>
+constexpr int lds_pad(int P) { return P + (sizeof(R) == 4 && P == 8 ? 2 : 1); }
+
 // Stockham pass of radix RADIX on P pencils (see lds_fft).
 template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
 __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, int Ns, const cplx<R>* __restrict__ tw) {
-    constexpr int PP = P + 1, M = NC / RADIX;
+    constexpr int PP = lds_pad<R>(P), M = NC / RADIX;
     const int s = TWS * (NC / (RADIX * Ns));
     for (int t = threadIdx.x; t < P * M; t += kThreads) {
         const int c = t % P, j = t / P;
@@ -150,7 +159,7 @@ __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, 
 // the DFT in registers, and writes back after a barrier.
 template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
 __device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, int Ns, const cplx<R>* __restrict__ tw) {
-    constexpr int PP = P + 1, M = NC / RADIX, T = P * M, IT = (T + kThreads - 1) / kThreads;
+    constexpr int PP = lds_pad<R>(P), M = NC / RADIX, T = P * M, IT = (T + kThreads - 1) / kThreads;
     const int s = TWS * (NC / (RADIX * Ns));
     cplx<R> v[IT][RADIX];
 #pragma unroll
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
                                                     cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
                                                     double* __restrict__ bmax, unsigned long long* __restrict__ nfar,
                                                     double qscale, double nmean) {
-    constexpr int NC = N / 2, P = Plan<R, N, V>::PZ, PP = P + 1;
+    constexpr int NC = N / 2, P = Plan<R, N, V>::PZ, PP = lds_pad<R>(P);
     __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
     double lmax = 0.0;
@@ -273,7 +282,7 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
                                                     size_t cs, size_t cn, const cplx<R>* __restrict__ tw,
                                                     const double* __restrict__ bmax, int nbmax,
                                                     unsigned long long* __restrict__ slot, XMap xm) {
-    constexpr int P = Plan<R, N, V>::PY, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
+    constexpr int P = Plan<R, N, V>::PY, PP = lds_pad<R>(P), NH = N / 2, NBM = N * (NH / P);
     constexpr int ML2 = radix_l2<R, INV ? PK_YINV : PK_YFWD>();
     constexpr bool IP = Plan<R, N, V>::INPLACE;
     __shared__ cplx<R> A[N * PP], B[IP ? 1 : N * PP];
@@ -349,7 +358,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
         sm2 = tab[k].sm2;
         smooth_on = sm2 > 0.0 ? 1 : 0;
     }
-    constexpr int P = Plan<R, N, V>::PX, PP = P + 1, NH = N / 2, NBM = N * (NH / P);
+    constexpr int P = Plan<R, N, V>::PX, PP = lds_pad<R>(P), NH = N / 2, NBM = N * (NH / P);
     constexpr int EPT = (N * P + kThreads - 1) / kThreads;
     constexpr int ML2 = radix_l2<R, PK_X>();
     constexpr bool IP = Plan<R, N, V>::INPLACE;
@@ -491,7 +500,7 @@ template <typename R, int N, int NCOMP, int V>
 __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ gspec, const cplx<R>* __restrict__ gnyq,
                                                     size_t gcs, size_t gcn, R* __restrict__ out, size_t ocs,
                                                     const cplx<R>* __restrict__ tw, R scale) {
-    constexpr int NC = N / 2, P = Plan<R, N, V>::template pc<NCOMP>(), NP = NCOMP * P, PP = NP + 1;
+    constexpr int NC = N / 2, P = Plan<R, N, V>::template pc<NCOMP>(), NP = NCOMP * P, PP = lds_pad<R>(NP);
     __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
     for (int t = threadIdx.x; t < NP * NC; t += kThreads) {
