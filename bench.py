@@ -193,6 +193,7 @@ def main():
         "kernels": kernels,
         "step_kernel_sum_us": tot / args.steps * 1e3,
         "fft_tiles": pm.fft_tiles(),
+        "deposit_tiles": list(pm.deposit_tiles()),
     }
     print(json.dumps(out), flush=True)
 

@@ -143,6 +143,14 @@ int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, 
  * passes.  No reference counterpart (numpy's FFT has no tiles). */
 int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax);
 
+/* Column shape of the deposit: shape[0..2] = BX, BY, N of the owner-computes columns (BX x BY x N
+ * cells per workgroup), all 0 for the atomic path (Np != N or N < 16).  pm3d_create times the 8x8,
+ * 4x8, 4x4 and 2x4 columns that fit in LDS on a jittered lattice and keeps the fastest (cached per
+ * mesh size and precision); every shape gives the same mesh bits.  Environment PM3D_DEPOSIT_SHAPE =
+ * one digit 0-3 forces the n-th fitting shape.  Returns 1 for columns, 0 for the atomic path.  No
+ * reference counterpart (pm3d.py:70-89 deposits with numpy.add.at). */
+int pm3d_deposit_tiles(void* handle, int32_t* shape);
+
 /* Initial conditions built on the device (the construction of PM3D.make_ic, pm3d.py:59-68,
  * generalised to an explicit linear density field at a_i): delta = mu*exp(-0.5*sum (x_c/s_c)^2)
  * around the box-centre vertex, plus optionally a Gaussian random field with sqrt(P(k)) = 1/|k| for

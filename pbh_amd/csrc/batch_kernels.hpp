@@ -62,6 +62,11 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox*
     deposit_far_seg_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, seg, b.N);
 }
 
+__global__ __launch_bounds__(kThreads) void k_deposit_far_bal_boxes(const MxBox* __restrict__ boxes, int ncol, int seg) {
+    const MxBox& b = boxes[blockIdx.y];
+    deposit_far_bal_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, ncol, seg, b.N, blockIdx.x, gridDim.x);
+}
+
 template <int N>
 __global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];

@@ -8,7 +8,9 @@
 // pass).  This relies on Np == N and on each particle's base cell being its site or the site just
 // below it on every axis ("near").  A particle outside that range ("far") is recorded by the
 // workgroup that owns its site, in that workgroup's own segment of `far` (no global counter), and is
-// added afterwards by k_deposit_far_seg with global atomics (several workgroups per segment).
+// added afterwards with global atomics by k_deposit_far_bal (the lists of all columns dealt out evenly
+// over the workgroups) or, for more columns than its prefix holds, k_deposit_far_seg (several
+// workgroups per column).
 //
 // Each corner weight ((wx*wy)*wz, the reference's order) is rounded to the fixed-point grid on its
 // own, so every path that deposits a particle adds the same integers; integer accumulation
@@ -268,6 +270,22 @@ __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long*
     deposit_col_body<Src, BX, BY, N, WIDE, NT>(src, mesh, far, farcnt, xm);
 }
 
+// One far particle's eight corners, added with global atomics.
+template <class Src, bool WIDE>
+__device__ __forceinline__ void far_add(const Src& src, unsigned long long* __restrict__ mesh, long long i, int N) {
+    int lo[3], hi[3];
+    typename Src::W w0[3], w1[3];
+    src.axes(src.load(i), lo, hi, w0, w1);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const unsigned long long v =
+            src.quant(((c >> 2 ? w1[0] : w0[0]) * ((c >> 1) & 1 ? w1[1] : w0[1])) * (c & 1 ? w1[2] : w0[2]));
+        if (v == 0ull) continue;
+        const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
+        fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, v);
+    }
+}
+
 // Far particles, added with global atomics: workgroup (b, y) takes every gridDim.y-th block of
 // kThreads entries of column b's list (a list holds at most the column's own sites).
 constexpr int kFarSub = 4;
@@ -277,20 +295,8 @@ __device__ __forceinline__ void deposit_far_seg_body(Src src, unsigned long long
                                                      const unsigned int* __restrict__ farcnt, int seg, int N) {
     const unsigned int cnt = farcnt[blockIdx.x];
     const int* list = far + (size_t)blockIdx.x * seg;
-    for (unsigned int t = blockIdx.y * kThreads + threadIdx.x; t < cnt; t += gridDim.y * kThreads) {
-        const long long i = list[t];
-        int lo[3], hi[3];
-        typename Src::W w0[3], w1[3];
-        src.axes(src.load(i), lo, hi, w0, w1);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const unsigned long long v = src.quant(((c >> 2 ? w1[0] : w0[0]) * ((c >> 1) & 1 ? w1[1] : w0[1])) *
-                                                   (c & 1 ? w1[2] : w0[2]));
-            if (v == 0ull) continue;
-            const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-            fx_atomic_add<WIDE>(mesh, ((size_t)ix * N + iy) * N + iz, v);
-        }
-    }
+    for (unsigned int t = blockIdx.y * kThreads + threadIdx.x; t < cnt; t += gridDim.y * kThreads)
+        far_add<Src, WIDE>(src, mesh, list[t], N);
 }
 
 template <class Src, bool WIDE>
@@ -300,16 +306,115 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned 
     deposit_far_seg_body<Src, WIDE>(src, mesh, far, farcnt, seg, N);
 }
 
-// Column shape per mesh size, cell width and LDS budget: the widest of 8x8, 4x8, 4x4, 2x4
-// columns whose LDS accumulator (rows of N+1 cells) fits in BUDGET bytes (gfx950 has 160 KB of LDS
-// per CU).
-template <int N, bool WIDE, int BUDGET = 135168> struct ColShape {
+// Far particles, load-balanced: the far particles of a collapsing peak crowd into the few columns
+// around it, so splitting per column leaves most workgroups idle while a handful walk long lists.
+// Here every workgroup scans the ncol column counts into an exclusive prefix in LDS, and the
+// concatenation of all lists is dealt out in blocks of kThreads entries, block k to workgroup
+// k mod gridDim.x; an entry's column is found by binary search of the prefix.
+constexpr int kFarColsMax = 8192;  // columns the prefix holds (32 KB of LDS)
+constexpr int kFarBalGrid = 1024;  // workgroups of the balanced pass (at most)
+template <class Src, bool WIDE>
+__device__ __forceinline__ void deposit_far_bal_body(Src src, unsigned long long* __restrict__ mesh,
+                                                     const int* __restrict__ far,
+                                                     const unsigned int* __restrict__ farcnt, int ncol, int seg,
+                                                     int N, unsigned blk, unsigned nblk) {
+    __shared__ unsigned int pre[kFarColsMax + 1];
+    __shared__ unsigned int wsum[kThreads / 64];
+    const int R = (ncol + kThreads - 1) / kThreads;  // columns per thread, contiguous
+    const int c0 = (int)threadIdx.x * R;
+    unsigned int sum = 0;
+    for (int k = 0; k < R; ++k)
+        if (c0 + k < ncol) sum += farcnt[c0 + k];
+    // exclusive block scan of the per-thread sums
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned int inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    unsigned int run = inc - sum;
+    for (int w = 0; w < wv; ++w) run += wsum[w];
+    for (int k = 0; k < R; ++k)
+        if (c0 + k < ncol) {
+            pre[c0 + k] = run;
+            run += farcnt[c0 + k];
+        }
+    if (threadIdx.x == kThreads - 1) pre[ncol] = run;  // the last thread's run ends at the total
+    __syncthreads();
+    const unsigned int total = pre[ncol];
+    for (unsigned int t = blk * kThreads + threadIdx.x; t < total; t += nblk * kThreads) {
+        int a = 0, b = ncol - 1;  // last column with pre[c] <= t (columns with no entries are skipped)
+        while (a < b) {
+            const int m = (a + b + 1) >> 1;
+            if (pre[m] <= t) a = m;
+            else b = m - 1;
+        }
+        far_add<Src, WIDE>(src, mesh, far[(size_t)a * seg + (t - pre[a])], N);
+    }
+}
+
+template <class Src, bool WIDE>
+__global__ __launch_bounds__(kThreads) void k_deposit_far_bal(Src src, unsigned long long* __restrict__ mesh,
+                                                              const int* __restrict__ far,
+                                                              const unsigned int* __restrict__ farcnt, int ncol,
+                                                              int seg, int N) {
+    deposit_far_bal_body<Src, WIDE>(src, mesh, far, farcnt, ncol, seg, N, blockIdx.x, gridDim.x);
+}
+
+// Column shape per mesh size, cell width and LDS budget.  The candidates are the 8x8, 4x8, 4x4 and
+// 2x4 columns, widest first; those whose LDS accumulator (rows of N+1 cells) fits in BUDGET bytes
+// (gfx950 has 160 KB of LDS per CU) are variants 0, 1, ... (NV of them; a larger V selects the last).
+// Variant 0 has the least ring traffic per owned site ((BX+2)(BY+2)/(BX*BY) sites read per site);
+// the narrower ones launch more workgroups and fit more of them per CU.  All variants give the same
+// mesh bits (integer sums), so the choice is made by timing.
+template <int N, bool WIDE, int V = 0, int BUDGET = 135168> struct ColShape {
     static constexpr int CELLB = WIDE ? 16 : 8;
     static constexpr bool fits(int bx, int by) { return bx * by * (N + 1) * CELLB <= BUDGET; }
-    static constexpr int BX = fits(8, 8) ? 8 : fits(4, 8) ? 4 : fits(4, 4) ? 4 : 2;
-    static constexpr int BY = fits(8, 8) ? 8 : fits(4, 8) ? 8 : fits(4, 4) ? 4 : 4;
+    static constexpr int kBX[4] = {8, 4, 4, 2}, kBY[4] = {8, 8, 4, 4};
+    static constexpr int first() {
+        for (int k = 0; k < 3; ++k)
+            if (fits(kBX[k], kBY[k])) return k;
+        return 3;
+    }
+    static constexpr int NV = 4 - first();
+    static constexpr int K = first() + (V < NV ? V : NV - 1);
+    static constexpr int BX = kBX[K], BY = kBY[K];
     static constexpr int NBLK = (N / BX) * (N / BY);
     static_assert(BX * BY * (N + 1) * CELLB <= 163840, "column does not fit in LDS");
+    static_assert(N % BX == 0 && N % BY == 0, "columns must tile the mesh");
 };
+
+// Timing state for choosing the column shape: the particle of lattice site (s0, s1, s2) at cell
+// coordinate g_d = s_d + j_d (g = x/dx - 0.5), j_d in [-0.45, 0.45) from a hash of (i, d), so every
+// particle is near its own column and the columns carry the lattice's full load.  FP64 positions
+// x = (g + 0.5) dx stay inside (0, L); the mixed state is the cell floor(g) mod N and offset g - floor(g).
+__device__ __forceinline__ double tune_jitter(unsigned long long i, int d) {
+    unsigned long long z = i * 3ull + (unsigned long long)d + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return ((double)(z >> 40) * 0x1p-24 - 0.5) * 0.9;
+}
+__global__ __launch_bounds__(kThreads) void k_tune_lattice(int N, double dx, double* __restrict__ x,
+                                                           int* __restrict__ cell, float* __restrict__ off) {
+    const long long n = (long long)N * N * N;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int s[3] = {(int)(i / ((long long)N * N)), (int)((i / N) % N), (int)(i % N)};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double g = s[d] + tune_jitter((unsigned long long)i, d);
+        if (x) {
+            x[d * n + i] = (g + 0.5) * dx;
+        } else {
+            const double fl = floor(g);
+            cell[d * n + i] = fl < 0.0 ? N - 1 : (int)fl;
+            off[d * n + i] = (float)(g - fl);
+        }
+    }
+}
 
 }  // namespace pm3d

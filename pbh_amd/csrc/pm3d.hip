@@ -110,6 +110,7 @@ struct Handle {
     std::string err;
     size_t bytes = 0;
     int fftv[5] = {};  // FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z)
+    int depv = 0;      // column shape variant of the deposit (ColShape<N, WIDE, V>)
     int bmax_pending = 0;  // per-block deposit maxima left in bmax by the plane forward pass
     // analysis sets (pm3d_set_analysis_sets): indices, offsets, centre/moment/partial buffers
     int* an_idx = nullptr;
@@ -526,44 +527,96 @@ SrcMx src_mx(const Handle* h) {
     return SrcMx{h->cell, h->cell + h->n, h->cell + 2 * h->n, h->off, h->off + h->n, h->off + 2 * h->n, h->N};
 }
 
-// Column deposit (deposit.hpp) for one mesh size.
-template <int N, bool WIDE, class Src>
+// Workgroups of the balanced far pass: enough for every particle to be far in one sweep of up to
+// kFarBalGrid workgroups.
+unsigned far_bal_grid(long long n) {
+    return (unsigned)std::min<long long>(kFarBalGrid, (n + kThreads - 1) / kThreads);
+}
+
+// Column deposit (deposit.hpp) for one mesh size and column shape variant.
+template <int N, bool WIDE, int V, class Src>
 void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh) {
-    using S = ColShape<N, WIDE>;
+    using S = ColShape<N, WIDE, V>;
     {
         Timed t(h, KC_DEPOSIT);
         k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024><<<S::NBLK, 1024, 0, h->st>>>(
             src, mesh, h->far, h->farcnt, XMap{1, 1});
     }
     Timed t(h, KC_DEPOSIT_FAR);
-    // a column lists at most its own BX*BY*N sites: kFarSub workgroups per column bound the work of
-    // the fullest one
-    k_deposit_far_seg<Src, WIDE><<<dim3(S::NBLK, kFarSub), kThreads, 0, h->st>>>(src, mesh, h->far, h->farcnt,
-                                                                              S::BX * S::BY * N, N);
+    if constexpr (S::NBLK <= kFarColsMax) {
+        k_deposit_far_bal<Src, WIDE><<<far_bal_grid(h->n), kThreads, 0, h->st>>>(src, mesh, h->far, h->farcnt, S::NBLK,
+                                                                                S::BX * S::BY * N, N);
+    } else {
+        // a column lists at most its own BX*BY*N sites: kFarSub workgroups per column bound the work
+        // of the fullest one
+        k_deposit_far_seg<Src, WIDE><<<dim3(S::NBLK, kFarSub), kThreads, 0, h->st>>>(src, mesh, h->far, h->farcnt,
+                                                                                  S::BX * S::BY * N, N);
+    }
+}
+
+template <int N, bool WIDE, class Src>
+void deposit_columns_v(Handle* h, const Src& src, unsigned long long* mesh, int v) {
+    switch (v) {
+        case 1: deposit_columns<N, WIDE, 1>(h, src, mesh); break;
+        case 2: deposit_columns<N, WIDE, 2>(h, src, mesh); break;
+        case 3: deposit_columns<N, WIDE, 3>(h, src, mesh); break;
+        default: deposit_columns<N, WIDE, 0>(h, src, mesh); break;
+    }
 }
 
 template <bool WIDE, class Src>
-int deposit_columns_n(Handle* h, const Src& src, unsigned long long* mesh) {
+int deposit_columns_n(Handle* h, const Src& src, unsigned long long* mesh, int v) {
     switch (h->N) {
-        case 16: deposit_columns<16, WIDE>(h, src, mesh); break;
-        case 32: deposit_columns<32, WIDE>(h, src, mesh); break;
-        case 48: deposit_columns<48, WIDE>(h, src, mesh); break;
-        case 64: deposit_columns<64, WIDE>(h, src, mesh); break;
-        case 96: deposit_columns<96, WIDE>(h, src, mesh); break;
-        case 128: deposit_columns<128, WIDE>(h, src, mesh); break;
-        case 256: deposit_columns<256, WIDE>(h, src, mesh); break;
-        case 512: deposit_columns<512, WIDE>(h, src, mesh); break;
+        case 16: deposit_columns_v<16, WIDE>(h, src, mesh, v); break;
+        case 32: deposit_columns_v<32, WIDE>(h, src, mesh, v); break;
+        case 48: deposit_columns_v<48, WIDE>(h, src, mesh, v); break;
+        case 64: deposit_columns_v<64, WIDE>(h, src, mesh, v); break;
+        case 96: deposit_columns_v<96, WIDE>(h, src, mesh, v); break;
+        case 128: deposit_columns_v<128, WIDE>(h, src, mesh, v); break;
+        case 256: deposit_columns_v<256, WIDE>(h, src, mesh, v); break;
+        case 512: deposit_columns_v<512, WIDE>(h, src, mesh, v); break;
         default: return fail(h, PM3D_EUNSUPPORTED, "column deposit needs N in [16,512]");
     }
     return 0;
 }
+
+// Column shape (BX, BY) of variant v for this handle's mesh size and cell width.
+template <int N, bool WIDE>
+void col_shape_v(int v, int& bx, int& by) {
+    switch (v) {
+        case 1: bx = ColShape<N, WIDE, 1>::BX, by = ColShape<N, WIDE, 1>::BY; break;
+        case 2: bx = ColShape<N, WIDE, 2>::BX, by = ColShape<N, WIDE, 2>::BY; break;
+        case 3: bx = ColShape<N, WIDE, 3>::BX, by = ColShape<N, WIDE, 3>::BY; break;
+        default: bx = ColShape<N, WIDE, 0>::BX, by = ColShape<N, WIDE, 0>::BY; break;
+    }
+}
+template <bool WIDE>
+void col_shape_n(int N, int v, int& bx, int& by) {
+    bx = by = 0;
+    switch (N) {
+        case 16: col_shape_v<16, WIDE>(v, bx, by); break;
+        case 32: col_shape_v<32, WIDE>(v, bx, by); break;
+        case 48: col_shape_v<48, WIDE>(v, bx, by); break;
+        case 64: col_shape_v<64, WIDE>(v, bx, by); break;
+        case 96: col_shape_v<96, WIDE>(v, bx, by); break;
+        case 128: col_shape_v<128, WIDE>(v, bx, by); break;
+        case 256: col_shape_v<256, WIDE>(v, bx, by); break;
+        case 512: col_shape_v<512, WIDE>(v, bx, by); break;
+        default: break;
+    }
+}
+void col_shape(const Handle* h, int v, int& bx, int& by) {
+    if (h->wide) col_shape_n<true>(h->N, v, bx, by);
+    else col_shape_n<false>(h->N, v, bx, by);
+}
+bool columns_used(const Handle* h) { return h->Np == h->N && h->N >= 16; }
 
 // Deposit of the device state into h->mesh.  The column path (Np == N, N >= 16) writes every cell;
 // the atomic path clears the mesh first.  This deposit's maximum is stored into its ring slot by the
 // y pass of the FFT.
 // Deposit of the device state into `mesh` (h->mesh, or a scratch mesh of the same size).
 int deposit_into(Handle* h, unsigned long long* mesh) {
-    const bool tiled = h->Np == h->N && h->N >= 16;  // a column plus its one-site ring must not wrap onto itself
+    const bool tiled = columns_used(h);  // a column plus its one-site ring must not wrap onto itself
     const int W = h->wide ? 2 : 1;
     if (!tiled) {
         Timed t(h, KC_FILL);
@@ -572,7 +625,7 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
     int rc = 0;
     if (!h->mixed) {
         if (tiled) {
-            rc = deposit_columns_n<true>(h, src_f64(h), mesh);
+            rc = deposit_columns_n<true>(h, src_f64(h), mesh, h->depv);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic<true>
@@ -580,7 +633,7 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
         }
     } else {
         if (tiled) {
-            rc = deposit_columns_n<false>(h, src_mx(h), mesh);
+            rc = deposit_columns_n<false>(h, src_mx(h), mesh, h->depv);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, mesh);
@@ -592,6 +645,73 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
 int deposit_state(Handle* h) {
     h->serial += 1;
     return deposit_into(h, h->mesh);
+}
+
+// Picks the column shape of the deposit (ColShape variants: 8x8, 4x8, 4x4, 2x4 columns, those that
+// fit in LDS) by timing the column and far passes of each (5 launches after one warm-up) on a
+// jittered lattice written into this handle's position buffers (k_tune_lattice; pm3d_set_state or
+// pm3d_make_peak_ic replaces it).  Every shape gives the same mesh bits, so the choice never changes
+// a result.  Cached per (N, precision) for the process.  PM3D_DEPOSIT_SHAPE = one digit 0-3 forces
+// the variant.
+std::map<std::pair<int, bool>, int> g_dep_tuned;
+
+int tune_deposit(Handle* h) {
+    h->depv = 0;
+    if (!columns_used(h)) return 0;
+    const char* env = std::getenv("PM3D_DEPOSIT_SHAPE");
+    if (env && *env && std::strcmp(env, "auto") != 0) {
+        if (std::strlen(env) != 1 || env[0] < '0' || env[0] > '3')
+            return fail(h, PM3D_EINVAL, "PM3D_DEPOSIT_SHAPE must be auto or one digit 0-3");
+        h->depv = env[0] - '0';
+        return 0;
+    }
+    const std::pair<int, bool> key{h->N, h->mixed};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_dep_tuned.find(key);
+        if (it != g_dep_tuned.end()) {
+            h->depv = it->second;
+            return 0;
+        }
+    }
+    k_tune_lattice<<<grid_for(h->n), kThreads, 0, h->st>>>(h->N, h->dx, h->mixed ? nullptr : h->x, h->cell, h->off);
+    int rc = check_launch(h, "deposit shape timing state");
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HCHK(h, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return fail(h, PM3D_EHIP, "hipEventCreate failed");
+    }
+    auto launch = [&](int v) {
+        return h->mixed ? deposit_columns_n<false>(h, src_mx(h), h->mesh, v)
+                        : deposit_columns_n<true>(h, src_f64(h), h->mesh, v);
+    };
+    int best = 0, pbx = -1, pby = -1;
+    float best_ms = 0.f;
+    for (int v = 0; v < 4 && !rc; ++v) {
+        int bx, by;
+        col_shape(h, v, bx, by);
+        if (bx == pbx && by == pby) break;  // variants past the last fitting shape repeat it
+        pbx = bx, pby = by;
+        float ms = 0.f;
+        if ((rc = launch(v))) break;
+        if (hipEventRecord(e0, h->st) != hipSuccess) { rc = fail(h, PM3D_EHIP, "hipEventRecord failed"); break; }
+        for (int r = 0; r < 5 && !rc; ++r) rc = launch(v);
+        if (rc) break;
+        if (hipEventRecord(e1, h->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(h, PM3D_EHIP, "deposit shape timing failed");
+        if (!rc) rc = check_launch(h, "deposit shape timing");
+        if (!rc && (v == 0 || ms < best_ms)) best = v, best_ms = ms;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    h->depv = best;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_dep_tuned[key] = best;
+    return 0;
 }
 
 // Gather blocks of kThreads consecutive particles: one x plane of the Lagrangian lattice is
@@ -846,7 +966,10 @@ void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
     const unsigned g = grid_for(h0->n);
     k_kick_drift_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev);
     k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, XMap{1, 1});
-    k_deposit_far_seg_boxes<<<dim3(S::NBLK, kFarSub, nb), kThreads, 0, st>>>(dev, S::BX * S::BY * N);
+    if constexpr (S::NBLK <= kFarColsMax)
+        k_deposit_far_bal_boxes<<<dim3(far_bal_grid(h0->n), nb), kThreads, 0, st>>>(dev, S::NBLK, S::BX * S::BY * N);
+    else
+        k_deposit_far_seg_boxes<<<dim3(S::NBLK, kFarSub, nb), kThreads, 0, st>>>(dev, S::BX * S::BY * N);
     k_fwd_plane_boxes<N><<<dim3(2 * N, nb), PlaneTile<float, N>::NT, 0, st>>>(dev, xmap(2 * N, 2));
     switch (h0->fftv[FP_X]) {
 #define PM3D_XBOX(V)                                                                                    \
@@ -1013,7 +1136,7 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         h->err = "initial copies failed";
         return bail(PM3D_EHIP);
     }
-    if ((rc = tune_fft(h))) return bail(rc);
+    if ((rc = tune_fft(h)) || (rc = tune_deposit(h))) return bail(rc);
     // the timing launches wrote a deposit maximum into the ring
     if (hipMemsetAsync(h->dscal, 0, (8 + kRing) * sizeof(unsigned long long), h->st) != hipSuccess ||
         hipStreamSynchronize(h->st) != hipSuccess) {
@@ -1434,6 +1557,19 @@ int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax) {
     Handle* h = (Handle*)handle;
     for (int p = 0; p < FP_COUNT && p < nmax; ++p) variants[p] = h->fftv[p];
     return FP_COUNT;
+}
+
+int pm3d_deposit_tiles(void* handle, int32_t* shape) {
+    if (!handle || !shape) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    shape[0] = shape[1] = shape[2] = 0;
+    if (!columns_used(h)) return 0;
+    int bx, by;
+    col_shape(h, h->depv, bx, by);
+    shape[0] = bx;
+    shape[1] = by;
+    shape[2] = h->N;
+    return 1;
 }
 
 int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic) {

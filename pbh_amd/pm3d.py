@@ -410,6 +410,12 @@ class PM3D:
         check(self._L.pm3d_fft_tiles(self._h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 5), self._h)
         return dict(zip(("r2c_z", "y_fwd", "x", "y_inv", "c2r_z"), (int(t) for t in v)))
 
+    def deposit_tiles(self) -> tuple:
+        """Deposit column shape (BX, BY, N) chosen at creation; (0, 0, 0) for the atomic deposit."""
+        v = np.zeros(3, np.int32)
+        check(self._L.pm3d_deposit_tiles(self._h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), self._h)
+        return tuple(int(t) for t in v)
+
     def timings(self) -> dict:
         names = self._L.pm3d_kernel_names().decode().split(",")
         cnt = np.zeros(len(names), np.int64)

@@ -84,7 +84,8 @@ class RunRecord:
         self.record["peak_rss_MB"] = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1024.0
         if pm is not None:
             self.record["gpu"] = {**pm.device_info(), "hbm_held_bytes": pm.device_bytes(),
-                                  "precision": pm.precision, "fft_tiles": pm.fft_tiles()}
+                                  "precision": pm.precision, "fft_tiles": pm.fft_tiles(),
+                                  "deposit_tiles": list(pm.deposit_tiles())}
         if extra:
             self.record.update(extra)
         with open(self.path, "w") as fh:

@@ -314,3 +314,32 @@ def test_fft_tile_variants(PM3D, N, prec, monkeypatch):
     for t in ("1", "2", "3"):
         for u, v in zip(outs["0"], outs[t]):
             assert np.array_equal(u, v), t
+
+
+@pytest.mark.parametrize("N,prec", [(128, "fp64"), (256, "mixed"), (96, "fp64"), (64, "mixed"), (16, "fp64")])
+def test_deposit_shape_variants(PM3D, N, prec, monkeypatch):
+    """Every deposit column shape sums the same fixed-point corner weights, so a step's positions,
+    momenta and density contrast agree bit for bit across the shapes that fit, and the automatic
+    choice is one of them."""
+    from pbh_amd import ic
+    say("deposit shapes", N, prec)
+    spec = ic.PeakSpec(N=N, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    outs, shapes = {}, {}
+    for v in ("0", "1", "2", "3", "auto"):
+        monkeypatch.setenv("PM3D_DEPOSIT_SHAPE", v)
+        pm = PM3D(N, 1.0, N, precision=prec)
+        shapes[v] = pm.deposit_tiles()
+        pm.set_state(st["x"], st["p"])
+        a = st["a_i"]
+        for _ in range(3):
+            a, dm = pm.step(a, 0.02 * a)
+        outs[v] = (pm.x, pm.p, np.asarray(dm))
+        del pm
+    cands = [shapes[v] for v in ("0", "1", "2", "3")]
+    assert cands[0][:2] in ((8, 8), (4, 8), (4, 4), (2, 4)) and all(s[2] == N for s in cands)
+    assert shapes["auto"] in cands
+    assert len(set(cands)) >= 2
+    for v in ("1", "2", "3", "auto"):
+        for u, w in zip(outs["0"], outs[v]):
+            assert np.array_equal(u, w), (v, shapes[v])
