@@ -6,6 +6,8 @@ Workload (default "peak128"): periodic box L=1, 128^3 mesh, 128^3 = 2,097,152 pa
 triaxial Gaussian peak mu=0.2, sigma=(0.10,0.09,0.08) L, Zel'dovich growing mode at a_i=0.01 in EdS,
 seed 1, FP64 (positions, momenta, FFT, forces; 128-bit fixed-point deposit at 2^-52).  A step is one KDK
 leapfrog step of PM3D.step (deposit, FFT Poisson, gather, kick, drift) with inputs resident in HBM.
+The step's gather runs at the start of the next step, fused with its kicks and drift, so each of
+the K timed steps runs one gather (the last warm-up step's at the start of the first).
 
 Prints one JSON line: value = particle-steps/s = n*K/wall over K timed steps; `roofline` for the
 kernel with the largest share of step time (algorithmic bytes per launch / average launch time
@@ -35,8 +37,8 @@ def algorithmic_bytes(N: int, n: int, prec: str) -> dict:
     output).  FP64: positions/momenta/accelerations 3 x 8 B SoA, 16 B fixed-point cells, complex128
     half spectra (Mh = N*N*(N/2+1) points), interleaved 3 x 8 B gradient mesh.  Mixed: positions
     3 x (4 B cell + 4 B offset), FP32 momenta/accelerations/spectra/gradients, 8 B cells.  The mixed
-    drift rewrites a particle's cell only when it moves to another cell, so kick_drift counts the
-    offsets (3 x 4 B) as its position traffic."""
+    drift rewrites a particle's cell only when it moves to another cell, so kick_drift and
+    gather_kick count the offsets (3 x 4 B) as the positions they write."""
     M = N ** 3
     Mh = N * N * (N // 2 + 1)
     r = 8 if prec == "fp64" else 4
@@ -54,6 +56,8 @@ def algorithmic_bytes(N: int, n: int, prec: str) -> dict:
         # z and y fused per x plane (N <= 128 FP64, N <= 256 FP32)
         "fwd_zy": q * M + c * Mh,                      # read mesh; write spectrum
         "inv_yz": 3 * c * Mh + 3 * r * M,              # read 3 spectra; write the gradient meshes
+        # the previous step's gather + kicks + drift: read x (24 B both modes), p, gradient; write x, p
+        "gather_kick": 24 * n + 3 * r * n + 3 * r * M + 2 * 3 * r * n,
     }
 
 

@@ -1,6 +1,6 @@
 // Box-grouped launches of the scheduled step for batches (gfx950).
 //
-// A batch replay runs one step of every member.  Launched member by member, each of the nine
+// A batch replay runs one step of every member.  Launched member by member, each of the eight
 // kernels of a step fills only a few CUs for the smaller boxes (a 32^3 deposit is 16 workgroups)
 // and costs its fixed start-up time once per box, and a captured graph runs its per-box branches one
 // kernel at a time.  Here the members of one mesh size share every launch: the grid is the one-box
@@ -46,9 +46,10 @@ __device__ __forceinline__ SrcMx box_src(const MxBox& b) {
     return SrcMx{b.cell, b.cell + b.n, b.cell + 2 * b.n, b.off, b.off + b.n, b.off + 2 * b.n, b.N};
 }
 
-__global__ __launch_bounds__(kThreads) void k_kick_drift_mx_boxes(const MxBox* __restrict__ boxes) {
+// The previous step's gather fused with this step's kicks and drift (gather_kick_drift_mx_body).
+__global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
-    kick_drift_mx_body(b.cell, b.off, b.p, b.acc, b.n, b.N, 1, 0.f, 0.f, 0.f, b.dscal + 1, b.sched, b.tab);
+    gather_kick_drift_mx_body(b.cell, b.off, b.p, b.n, b.N, b.grad, 1, 0.f, 0.f, 0.f, b.dscal + 1, b.sched, b.tab, xm);
 }
 
 template <int BX, int BY, int N>
@@ -90,11 +91,6 @@ __global__ __launch_bounds__((InvTile<float, N, Q>::NT)) void k_inv_plane_boxes(
     constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
     inv_plane_body<float, N, Q>(b.gspec, b.gspec + specN, tot, tot, b.grad, (size_t)N * N * N, b.tw,
                                 (float)(1.0 / ((double)N * N * N)), xm);
-}
-
-__global__ __launch_bounds__(kThreads) void k_gather3_mx_boxes(const MxBox* __restrict__ boxes, XMap xm) {
-    const MxBox& b = boxes[blockIdx.y];
-    gather3_mx_body(MxPos{b.cell, b.off, b.n}, b.N, b.grad, b.acc, xm);
 }
 
 __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx_boxes(const MxBox* __restrict__ boxes) {

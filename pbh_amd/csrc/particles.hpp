@@ -67,18 +67,13 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
 
 // ------------------------------------------------------------------------------------- gather
 // acc_c = sum over corners ((g_c*wx)*wy)*wz in the reference's order (pm3d.py:91-99,113), from the
-// component-major gradient mesh grad[c*N^3 + cell].  KICK: p += (c1*acc)/den (pm3d.py:127), and the
-// acceleration is always stored for the opening kick of the next step (pm3d.py:128).
-template <bool KICK>
-__global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__ x, long long n, int N, double dx,
-                                                      const double* __restrict__ grad, double* __restrict__ acc,
-                                                      double* __restrict__ p, double c1, double den, XMap xm) {
-    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
-    if (i >= n) return;
+// component-major gradient mesh grad[c*N^3 + cell], at position X.
+__device__ __forceinline__ void gather3_at(const double X[3], int N, double dx, const double* __restrict__ grad,
+                                           double a[3]) {
     int lo[3], hi[3];
     double f[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) cic_axis_fast(x[d * n + i], dx, N, lo[d], hi[d], f[d]);
+    for (int d = 0; d < 3; ++d) cic_axis_fast(X[d], dx, N, lo[d], hi[d], f[d]);
     const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
     const size_t M = (size_t)N * N * N;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -91,13 +86,26 @@ __global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__
         a1 = a1 + ((g[M] * u) * v) * w;
         a2 = a2 + ((g[2 * M] * u) * v) * w;
     }
-    acc[i] = a0;
-    acc[n + i] = a1;
-    acc[2 * n + i] = a2;
+    a[0] = a0;
+    a[1] = a1;
+    a[2] = a2;
+}
+
+// Gather of every particle into acc (SoA).  KICK: p += (c1*acc)/den (pm3d.py:127) as well.
+template <bool KICK>
+__global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__ x, long long n, int N, double dx,
+                                                      const double* __restrict__ grad, double* __restrict__ acc,
+                                                      double* __restrict__ p, double c1, double den, XMap xm) {
+    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const double X[3] = {x[i], x[n + i], x[2 * n + i]};
+    double a[3];
+    gather3_at(X, N, dx, grad, a);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d * n + i] = a[d];
     if (KICK) {
-        p[i] = p[i] + (c1 * a0) / den;
-        p[n + i] = p[n + i] + (c1 * a1) / den;
-        p[2 * n + i] = p[2 * n + i] + (c1 * a2) / den;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) p[d * n + i] = p[d * n + i] + (c1 * a[d]) / den;
     }
 }
 
@@ -150,6 +158,48 @@ __global__ __launch_bounds__(kThreads) void k_kick_drift(double* __restrict__ x,
             if (pend) pd = pd + (c1p * ak) / denp;
             pd = pd + (c1 * ak) / den1;
             const double xd = npy_rem_near(x[k] + (da * pd) / den_d, L);
+            p[k] = pd;
+            x[k] = xd;
+            ok &= isfinite(pd) && isfinite(xd);
+        }
+    }
+    const unsigned long long nb = __ballot(!ok);
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, (unsigned long long)__popcll(nb));
+}
+
+// The gather of the previous step's forces fused with this step's kicks and drift.  A step ends
+// with the gradient mesh of the forces at its new positions; the gather that turns it into the
+// acceleration is deferred to here, where the same positions are loaded once for the gather and
+// the drift: acc = gather3_at(x), then exactly k_kick_drift's arithmetic on it (pending closing
+// kick, opening kick, drift, each its own rounding).  The acceleration is never stored.
+__global__ __launch_bounds__(kThreads) void k_gather_kick_drift(double* __restrict__ x, double* __restrict__ p,
+                                                                long long n, int N, double dx,
+                                                                const double* __restrict__ grad, int pend, double c1p,
+                                                                double denp, double c1, double den1, double da,
+                                                                double den_d, double L,
+                                                                unsigned long long* __restrict__ bad,
+                                                                const DevSched* __restrict__ sch,
+                                                                const DevStep* __restrict__ tab, XMap xm) {
+    if (sch) {
+        const int k = sch->counter;
+        if (k >= sch->nsteps) return;
+        const DevStep e = tab[k];
+        pend = 1;
+        c1p = e.c1p; denp = e.denp; c1 = e.c1; den1 = e.den1; da = e.da; den_d = e.den_d;
+    }
+    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
+    bool ok = true;
+    if (i < n) {
+        const double X[3] = {x[i], x[n + i], x[2 * n + i]};
+        double a[3];
+        gather3_at(X, N, dx, grad, a);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const long long k = d * n + i;
+            double pd = p[k];
+            if (pend) pd = pd + (c1p * a[d]) / denp;
+            pd = pd + (c1 * a[d]) / den1;
+            const double xd = npy_rem_near(X[d] + (da * pd) / den_d, L);
             p[k] = pd;
             x[k] = xd;
             ok &= isfinite(pd) && isfinite(xd);

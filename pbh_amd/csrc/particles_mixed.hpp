@@ -77,14 +77,9 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic_mx(MxPos P, int N, 
     mx_scatter(lo, f, N, mesh);
 }
 
-// CIC gather of the component-major FP32 gradient mesh grad[c*N^3 + cell] into acc (SoA).
-__device__ __forceinline__ void gather3_mx_body(MxPos P, int N, const float* __restrict__ grad,
-                                                float* __restrict__ acc, XMap xm) {
-    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
-    if (i >= P.n) return;
-    int lo[3];
-    float f[3];
-    P.load(i, lo, f);
+// CIC gather of the component-major FP32 gradient mesh grad[c*N^3 + cell] at base cell lo, fraction f.
+__device__ __forceinline__ void gather3_mx_at(const int lo[3], const float f[3], int N, const float* __restrict__ grad,
+                                              float a[3]) {
     const int hx = lo[0] + 1 == N ? 0 : lo[0] + 1, hy = lo[1] + 1 == N ? 0 : lo[1] + 1, hz = lo[2] + 1 == N ? 0 : lo[2] + 1;
     const float wx[2] = {1.0f - f[0], f[0]}, wy[2] = {1.0f - f[1], f[1]}, wz[2] = {1.0f - f[2], f[2]};
     const size_t M = (size_t)N * N * N;
@@ -98,9 +93,22 @@ __device__ __forceinline__ void gather3_mx_body(MxPos P, int N, const float* __r
         a1 = fmaf(g[M], w, a1);
         a2 = fmaf(g[2 * M], w, a2);
     }
-    acc[i] = a0;
-    acc[P.n + i] = a1;
-    acc[2 * P.n + i] = a2;
+    a[0] = a0;
+    a[1] = a1;
+    a[2] = a2;
+}
+
+// Gather of every particle into acc (SoA).
+__device__ __forceinline__ void gather3_mx_body(MxPos P, int N, const float* __restrict__ grad,
+                                                float* __restrict__ acc, XMap xm) {
+    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
+    if (i >= P.n) return;
+    int lo[3];
+    float f[3], a[3];
+    P.load(i, lo, f);
+    gather3_mx_at(lo, f, N, grad, a);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d * P.n + i] = a[d];
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const float* __restrict__ grad,
@@ -164,6 +172,68 @@ __global__ __launch_bounds__(kThreads) void k_kick_drift_mx(int* __restrict__ ce
                                                             const DevSched* __restrict__ sch,
                                                             const DevStep* __restrict__ tab) {
     kick_drift_mx_body(cell, off, p, acc, n, N, pend, kp, k1, sd, bad, sch, tab);
+}
+
+// The gather of the previous step's forces fused with this step's kicks and drift (the mixed form
+// of k_gather_kick_drift): the cell and offset are loaded once, acc = gather3_mx_at, then exactly
+// kick_drift_mx_body's arithmetic.  The acceleration is never stored.
+__device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell, float* __restrict__ off,
+                                                          float* __restrict__ p, long long n, int N,
+                                                          const float* __restrict__ grad, int pend, float kp, float k1,
+                                                          float sd, unsigned long long* __restrict__ bad,
+                                                          const DevSched* __restrict__ sch,
+                                                          const DevStep* __restrict__ tab, XMap xm) {
+    if (sch) {
+        const int k = sch->counter;
+        if (k >= sch->nsteps) return;
+        pend = 1;
+        kp = tab[k].kp;
+        k1 = tab[k].k1;
+        sd = tab[k].sd;
+    }
+    const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
+    bool ok = true;
+    if (i < n) {
+        int lo[3];
+        float f[3], a[3];
+        MxPos{cell, off, n}.load(i, lo, f);
+        gather3_mx_at(lo, f, N, grad, a);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const long long k = d * n + i;
+            float pd = p[k];
+            if (pend) pd = fmaf(kp, a[d], pd);
+            pd = fmaf(k1, a[d], pd);
+            float fo = f[d] + pd * sd;
+            float fl = floorf(fo);
+            fo -= fl;
+            if (fo >= 1.0f) {  // a tiny negative offset rounds up to 1 after the subtraction: next cell, offset 0
+                fo = 0.0f;
+                fl += 1.0f;
+            }
+            p[k] = pd;
+            off[k] = fo;
+            ok &= isfinite(pd) && isfinite(fo) && fabsf(fl) < 1e6f;
+            if (fl != 0.0f && ok) {
+                int c = lo[d] + (int)fl;
+                c %= N;
+                if (c < 0) c += N;
+                cell[k] = c;
+            }
+        }
+    }
+    const unsigned long long nb = __ballot(!ok);
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, (unsigned long long)__popcll(nb));
+}
+
+__global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx(int* __restrict__ cell, float* __restrict__ off,
+                                                                   float* __restrict__ p, long long n, int N,
+                                                                   const float* __restrict__ grad, int pend, float kp,
+                                                                   float k1, float sd,
+                                                                   unsigned long long* __restrict__ bad,
+                                                                   const DevSched* __restrict__ sch,
+                                                                   const DevStep* __restrict__ tab, XMap xm) {
+    gather_kick_drift_mx_body(cell, off, p, n, N, grad, pend, kp, k1, sd, bad, sch, tab, xm);
 }
 
 __global__ __launch_bounds__(kThreads) void k_kick_mx(float* __restrict__ p, const float* __restrict__ acc, long long n3,

@@ -55,10 +55,11 @@ enum KernelClass {
     KC_OTHER,
     KC_FWD_ZY,
     KC_INV_YZ,
+    KC_GATHER_KICK,
     KC_COUNT
 };
 const char* kKernelNames =
-    "kick_drift,fill,deposit,deposit_far,r2c_z,y_fwd,x_poisson,y_inv,c2r_z,gather,other,fwd_zy,inv_yz";
+    "kick_drift,fill,deposit,deposit_far,r2c_z,y_fwd,x_poisson,y_inv,c2r_z,gather,other,fwd_zy,inv_yz,gather_kick";
 
 struct Handle {
     pm3d_cfg cfg{};
@@ -98,7 +99,8 @@ struct Handle {
     std::vector<DevStep> htab;
     bool use_sched = false;  // set while a batch graph is being captured
     bool state_set = false;
-    bool acc_valid = false;
+    bool acc_valid = false;   // acc holds the acceleration at the current positions
+    bool grad_valid = false;  // grad holds the force meshes of the last solve at the current positions
     bool kick_pending = false;
     double pend_c1 = 0, pend_den = 0;
     // timing
@@ -723,13 +725,17 @@ XMap gather_map(Handle* h) {
     return xmap(grid_for(h->n), share);
 }
 
-// forces at the current positions: deposit, Poisson, gather into acc_out (SoA 3n, state precision).
-int forces_state(Handle* h, double s, double sm2, void* acc_out) {
+// Force meshes at the current positions: deposit, Poisson, the three gradient meshes in h->grad.
+int solve_state(Handle* h, double s, double sm2) {
     int rc;
     if ((rc = deposit_state(h))) return rc;
     if ((rc = fft_forward(h, h->mesh, true))) return rc;
     if ((rc = fft_xpass(h, 2, s, sm2))) return rc;
-    if ((rc = fft_inverse(h, h->gspec, 3, h->grad))) return rc;
+    return fft_inverse(h, h->gspec, 3, h->grad);
+}
+
+// Gather of h->grad at the current positions into acc_out (SoA 3n, state precision).
+int gather_state(Handle* h, void* acc_out) {
     {
         Timed t(h, KC_GATHER);
         if (!h->mixed)
@@ -739,11 +745,28 @@ int forces_state(Handle* h, double s, double sm2, void* acc_out) {
             k_gather3_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, (const float*)h->grad, (float*)acc_out,
                                                                gather_map(h));
     }
-    return check_launch(h, "forces");
+    return check_launch(h, "gather");
+}
+
+// forces at the current positions: deposit, Poisson, gather into acc_out.
+int forces_state(Handle* h, double s, double sm2, void* acc_out) {
+    int rc = solve_state(h, s, sm2);
+    return rc ? rc : gather_state(h, acc_out);
+}
+
+// A step leaves the acceleration as force meshes (grad_valid, !acc_valid); the next step gathers
+// them in its first kernel.  Anything else that needs acc, or that replaces the force meshes or the
+// positions, gathers them into acc first.
+int settle_acc(Handle* h) {
+    if (!h->grad_valid || h->acc_valid) return 0;
+    int rc = gather_state(h, h->mixed ? (void*)h->accf : (void*)h->acc);
+    if (!rc) h->acc_valid = true;
+    return rc;
 }
 
 int flush_kick(Handle* h) {
     if (!h->kick_pending) return 0;
+    if (int rc = settle_acc(h)) return rc;
     Timed t(h, KC_OTHER);
     if (!h->mixed)
         k_kick<<<grid_for(3 * h->n), kThreads, 0, h->st>>>(h->p, h->acc, 3 * h->n, h->pend_c1, h->pend_den);
@@ -883,18 +906,20 @@ void spec_amp_n(Handle* h, double kcut) {
                                                                                        1.0 / ((double)N * h->dx), kcut);
 }
 
-// One step driven by the device schedule (for graph capture): kick+drift with the scheduled
-// scalars, forces, Jacobian minimum, schedule advance.  Launches only; no host synchronisation.
+// One step driven by the device schedule (for graph capture): the previous step's gather fused with
+// the kicks and drift of the scheduled scalars, force meshes, Jacobian minimum, schedule advance.
+// Launches only; no host synchronisation.
 int launch_sched_step(Handle* h) {
-    void* accbuf = h->mixed ? (void*)h->accf : (void*)h->acc;
     if (!h->mixed)
-        k_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->acc, h->n, 1, 0, 1, 0, 1, 0, 1, h->L,
-                                                             h->dscal + 1, h->sched, h->tab);
+        k_gather_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->n, h->N, h->dx, (const double*)h->grad,
+                                                                    1, 0, 1, 0, 1, 0, 1, h->L, h->dscal + 1, h->sched,
+                                                                    h->tab, gather_map(h));
     else
-        k_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(h->cell, h->off, h->pf, h->accf, h->n, h->N, 1, 0, 0, 0,
-                                                                h->dscal + 1, h->sched, h->tab);
+        k_gather_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(h->cell, h->off, h->pf, h->n, h->N,
+                                                                       (const float*)h->grad, 1, 0, 0, 0, h->dscal + 1,
+                                                                       h->sched, h->tab, gather_map(h));
     h->use_sched = true;
-    int rc = forces_state(h, 0.0, 0.0, accbuf);
+    int rc = solve_state(h, 0.0, 0.0);
     h->use_sched = false;
     if (rc) return rc;
     const unsigned g = grid_for(h->n);
@@ -958,13 +983,13 @@ MxBox box_of(Handle* h) {
     return b;
 }
 
-// One scheduled step of nb boxes of mesh size N in nine launches (the kernels and grids of
-// launch_sched_step per box, with the boxes along the grid's y (far pass: z) dimension).
+// One scheduled step of nb boxes of mesh size N in eight launches (the kernels and grids of
+// launch_sched_step per box, with the boxes along the grid's y dimension).
 template <int N>
 void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
     using S = ColShape<N, false>;
     const unsigned g = grid_for(h0->n);
-    k_kick_drift_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev);
+    k_gather_kick_drift_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev, gather_map(h0));
     k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, XMap{1, 1});
     if constexpr (S::NBLK <= kFarColsMax)
         k_deposit_far_bal_boxes<<<dim3(far_bal_grid(h0->n), nb), kThreads, 0, st>>>(dev, S::NBLK, S::BX * S::BY * N);
@@ -991,7 +1016,6 @@ void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
         constexpr int Q = Fft<float, N>::kInvQ;
         k_inv_plane_boxes<N, Q><<<dim3(3 * Q * N, nb), InvTile<float, N, Q>::NT, 0, st>>>(dev, xmap(3 * Q * N, Q));
     }
-    k_gather3_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev, gather_map(h0));
     k_min_jacobian_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev);
     k_sched_advance_boxes<<<dim3(1, nb), kThreads, 0, st>>>(dev, (int)g);
 }
@@ -1199,6 +1223,9 @@ int pm3d_set_state(void* handle, const double* x, const double* p, int64_t n) {
     int rc = ensure_sbuf(h, 3 * n * sizeof(double));
     if (rc) return rc;
     if (x) {
+        // the cached acceleration stays that of the old positions (as the reference's _acc)
+        if (!p && (rc = settle_acc(h))) return rc;
+        h->grad_valid = false;
         HCHK(h, hipMemcpyAsync(h->sbuf, x, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->st));
         if (!h->mixed)
             k_aos_to_soa<<<grid_for(n), kThreads, 0, h->st>>>(h->sbuf, h->x, n);
@@ -1346,6 +1373,8 @@ int pm3d_forces(void* handle, double a, double s, double sm2, double* acc, doubl
     int rc = ensure_sbuf(h, 3 * h->n * rb + std::max<size_t>(3 * h->n, h->M) * sizeof(double));
     if (rc) return rc;
     double* stage = (double*)((char*)h->sbuf + 3 * h->n * rb);
+    if ((rc = settle_acc(h))) return rc;  // the force meshes are about to be replaced
+    h->grad_valid = false;
     if ((rc = forces_state(h, s, sm2, h->sbuf))) return rc;
     if (acc) {
         if (!h->mixed)
@@ -1370,16 +1399,29 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
     Handle* h = (Handle*)handle;
     if (!h->state_set) return fail(h, PM3D_ESTATE, "state not set");
     int rc;
-    void* accbuf = h->mixed ? (void*)h->accf : (void*)h->acc;
-    if (sc->need_forces || !h->acc_valid) {
-        // the previous closing kick used the old acceleration: apply it before replacing acc
+    if (sc->need_forces || !(h->acc_valid || h->grad_valid)) {
+        // the previous closing kick used the old acceleration: apply it before replacing it
         if ((rc = flush_kick(h))) return rc;
-        if ((rc = forces_state(h, sc->s_old, sc->sm2, accbuf))) return rc;
-        h->acc_valid = true;
+        if ((rc = solve_state(h, sc->s_old, sc->sm2))) return rc;
+        h->acc_valid = false;
+        h->grad_valid = true;
     }
-    {
-        // pending closing kick of the previous step (pm3d.py:127) + opening kick (:122) + drift
+    if (h->grad_valid) {
+        // gather of the force meshes (pm3d.py:113, the previous step's or the solve above) +
+        // pending closing kick of the previous step (:127) + opening kick (:122) + drift
         // (:123-124) in one pass; FP64 keeps the reference's two separate roundings of p
+        Timed t(h, KC_GATHER_KICK);
+        if (!h->mixed)
+            k_gather_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(
+                h->x, h->p, h->n, h->N, h->dx, (const double*)h->grad, h->kick_pending ? 1 : 0, h->pend_c1, h->pend_den,
+                sc->c1, sc->den1, sc->da, sc->den_d, h->L, h->dscal + 1, nullptr, nullptr, gather_map(h));
+        else
+            k_gather_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(
+                h->cell, h->off, h->pf, h->n, h->N, (const float*)h->grad, h->kick_pending ? 1 : 0,
+                (float)(h->kick_pending ? h->pend_c1 / h->pend_den : 0.0), (float)(sc->c1 / sc->den1),
+                (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr, gather_map(h));
+    } else {
+        // the stored acceleration (a state whose force meshes were replaced after it was gathered)
         Timed t(h, KC_KICK_DRIFT);
         if (!h->mixed)
             k_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->acc, h->n, h->kick_pending ? 1 : 0,
@@ -1392,11 +1434,13 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
                 (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr);
     }
     h->kick_pending = false;
+    h->acc_valid = false;
+    h->grad_valid = false;
     if ((rc = check_launch(h, "kick_drift"))) return rc;
-    // the closing kick is deferred: the gather stores acc, and the kick is applied with the next
-    // opening kick or before p is read
-    if ((rc = forces_state(h, sc->s_new, sc->sm2, accbuf))) return rc;
-    h->acc_valid = true;
+    // the gather and the closing kick are deferred: the force meshes stay in h->grad, and the next
+    // step gathers them with its kicks (or they are gathered and the kick applied before p is read)
+    if ((rc = solve_state(h, sc->s_new, sc->sm2))) return rc;
+    h->grad_valid = true;
     h->kick_pending = true;
     h->pend_c1 = sc->c1;
     h->pend_den = sc->den2;
@@ -1425,6 +1469,8 @@ int pm3d_irfftn(void* handle, const double* in_c, double* out) {
 int pm3d_poisson_gradient(void* handle, const double* delta, double s, double sm2, double* g) {
     if (!handle || !delta || !g) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
+    if (int rc = settle_acc(h)) return rc;  // the force meshes are about to be replaced
+    h->grad_valid = false;
     return h->mixed ? poisson_impl<float>(h, delta, s, sm2, g) : poisson_impl<double>(h, delta, s, sm2, g);
 }
 
@@ -1620,6 +1666,7 @@ int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic) {
     HCHK(h, hipStreamSynchronize(h->st));
     h->state_set = true;
     h->acc_valid = false;
+    h->grad_valid = false;
     h->kick_pending = false;
     return check_launch(h, "make_peak_ic");
 }
@@ -1641,6 +1688,7 @@ int pm3d_copy_state(void* dst_handle, void* src_handle) {
     HCHK(d, hipStreamSynchronize(d->st));
     d->state_set = true;
     d->acc_valid = false;
+    d->grad_valid = false;
     d->kick_pending = false;
     (void)rc;
     return check_launch(d, "copy_state");
@@ -1649,7 +1697,7 @@ int pm3d_copy_state(void* dst_handle, void* src_handle) {
 int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps) {
     if (!handle || !table || nsteps < 1) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
-    if (!h->state_set || !h->acc_valid || !h->kick_pending)
+    if (!h->state_set || !h->grad_valid || !h->kick_pending)
         return fail(h, PM3D_ESTATE, "a schedule continues a stepped state: run one pm3d_step first");
     if (nsteps > h->tab_cap) {
         HCHK(h, hipStreamSynchronize(h->st));
@@ -1701,7 +1749,9 @@ int pm3d_schedule_end(void* handle, int* done) {
     int k = 0;
     int rc = pm3d_schedule_status(handle, &k, nullptr);
     if (rc) return rc;
-    if (k > 0) {  // the closing kick of the last scheduled step is pending
+    if (k > 0) {  // the gather and closing kick of the last scheduled step are pending
+        h->grad_valid = true;
+        h->acc_valid = false;
         h->kick_pending = true;
         h->pend_c1 = h->htab[k - 1].c1;
         h->pend_den = h->htab[k - 1].den2;
