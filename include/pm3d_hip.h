@@ -92,7 +92,10 @@ int pm3d_interp(void* handle, const double* F, const double* X, int64_t n, doubl
 int pm3d_forces(void* handle, double a, double s, double sm2, double* acc, double* delta);
 
 /* PM3D.step(a, da) (pm3d.py:117-129).  delta (N,N,N) and delta_max may be NULL; when both are
- * NULL the call is asynchronous (no host synchronisation). */
+ * NULL the call is asynchronous (no host synchronisation).  The step ends with the force meshes of
+ * the new positions on the device; their gather and the closing kick run at the start of the next
+ * step, fused with its opening kick and drift (same operations and order, same bits), or when p is
+ * read. */
 int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* delta_max);
 
 /* Lazy access to the density contrast of a deposit (the `delta` that PM3D.step returns,
@@ -145,8 +148,9 @@ int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax);
 
 /* Column shape of the deposit: shape[0..2] = BX, BY, N of the owner-computes columns (BX x BY x N
  * cells per workgroup), all 0 for the atomic path (Np != N or N < 16).  pm3d_create times the 8x8,
- * 4x8, 4x4 and 2x4 columns that fit in LDS on a jittered lattice and keeps the fastest (cached per
- * mesh size and precision); every shape gives the same mesh bits.  Environment PM3D_DEPOSIT_SHAPE =
+ * 4x8, 4x4 and 2x4 columns that fit in LDS on a jittered lattice and keeps the widest unless a
+ * narrower one is at least 3 % faster (cached per mesh size and precision); every shape gives the
+ * same mesh bits.  Environment PM3D_DEPOSIT_SHAPE =
  * one digit 0-3 forces the n-th fitting shape.  Returns 1 for columns, 0 for the atomic path.  No
  * reference counterpart (pm3d.py:70-89 deposits with numpy.add.at). */
 int pm3d_deposit_tiles(void* handle, int32_t* shape);
@@ -178,8 +182,8 @@ int pm3d_copy_state(void* dst_handle, void* src_handle);
  * evaluated in Python); entry k also carries the previous step's closing kick (c1p, denp).  The
  * device keeps a step counter; every scheduled step reads entry [counter], records the minimum of
  * det(dx/dq) of its new positions in a history and advances the counter.  A schedule continues a
- * stepped state (the opening acceleration and a pending closing kick must exist: run one
- * pm3d_step first).  Steps past the end of the table leave the state unchanged. */
+ * stepped state (the force meshes of the current positions and a pending closing kick must exist:
+ * run one pm3d_step first).  Steps past the end of the table leave the state unchanged. */
 typedef struct pm3d_sched_entry {
     double c1p, denp;  /* closing kick of the previous step: p += (c1p*acc)/denp */
     double c1, den1;   /* opening kick */

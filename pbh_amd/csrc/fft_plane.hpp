@@ -327,8 +327,10 @@ __device__ __forceinline__ void inv_plane_body(const cplx<R>* __restrict__ gspec
     const int comp = (int)(lin / (Q * N)), x = (int)((lin / Q) % N), q = (int)(lin % Q);
     const cplx<R>* sp = gspec + comp * gcs + (size_t)x * N * H;
     const cplx<R>* np = gnyq + comp * gcn + (size_t)x * N;
-    // each thread issues the loads of LB elements (Q spectrum values each) before it stores any
-    constexpr int IT = (RW * S + NT - 1) / NT, LB = (sizeof(R) == 8 ? 8 : 16) / Q > 0 ? (sizeof(R) == 8 ? 8 : 16) / Q : 1;
+    // each thread issues the loads of LB elements (Q spectrum values each) before it stores any: as
+    // few batches as keep the loaded data within VB VGPRs, balanced (one memory latency each)
+    constexpr int IT = (RW * S + NT - 1) / NT, VPE = Q * (int)sizeof(cplx<R>) / 4, VB = sizeof(R) == 8 ? 72 : 40;
+    constexpr int NB = (IT * VPE + VB - 1) / VB, LB = (IT + NB - 1) / NB;
     for (int i0 = 0; i0 < IT; i0 += LB) {
         cplx<R> a[LB][Q], w[LB];
 #pragma unroll
