@@ -336,7 +336,7 @@ __device__ __forceinline__ double khalf(int i, double kval) { return 6.283185307
 // forward FFT along x, then per mode  phik = ((s*dk) * (1/k2)) [* exp((-0.5*k2)*sm2)], phik(0)=0
 // (pm3d.py:105-109; numpy's complex/real division is a multiply by the reciprocal), then for
 // c = x,y,z: -1j*k_c*phik = (k_c*Im, -k_c*Re) (pm3d.py:112) and the inverse FFT along x, written to
-// the three gradient spectra.  If nbmax > 0, the last block also reduces per-block deposit maxima
+// the three gradient spectra (k_y and k_z are applied after the transform of phik, which is linear).  If nbmax > 0, the last block also reduces per-block deposit maxima
 // bmax[0..nbmax) into *slot (left there by the plane forward pass, fft_plane.hpp).
 template <typename R, int N, int MODE, int V>
 __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>* __restrict__ nyq,
@@ -456,23 +456,43 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
             }
         }
         __syncthreads();
-        for (int comp = 0; comp < 3; ++comp) {
+        // -1j*k_c for c = y, z is constant along an x pencil, so their inverse x transforms are
+        // -1j*k_c times the inverse transform of phik: pass 0 transforms phik once and writes both
+        // (the product rounded after the transform instead of before it), pass 1 transforms
+        // -1j*k_x*phik for the x component
+        for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll
             for (int e = 0; e < EPT; ++e) {
                 const int t = threadIdx.x + e * kThreads;
                 if (t < N * P) {
                     const int c = t % P, j = t / P;
-                    const int iy = main ? y : col0 + c, iz = main ? col0 + c : NH;
-                    const double kc = comp == 0 ? kfull(j, N, kval) : comp == 1 ? kfull(iy, N, kval) : khalf(iz, kval);
-                    A[j * PP + c] = cplx<R>{(R)(kc * (double)ph[e].im), (R)(-(kc * (double)ph[e].re))};
+                    if (pass == 0) {
+                        A[j * PP + c] = ph[e];
+                    } else {
+                        const double kx = kfull(j, N, kval);
+                        A[j * PP + c] = cplx<R>{(R)(kx * (double)ph[e].im), (R)(-(kx * (double)ph[e].re))};
+                    }
                 }
             }
             __syncthreads();
             const cplx<R>* r2 = lds_fft<R, N, P, true, 1, ML2, IP>(A, B, tw);
-            cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
-            for (int t = threadIdx.x; t < N * P; t += kThreads) {
-                const int c = t % P, j = t / P;
-                dst[off(j, c)] = r2[j * PP + c];
+            if (pass == 0) {
+                cplx<R>* d1 = main ? gspec + gcs : gnyq + gcn;
+                cplx<R>* d2 = main ? gspec + 2 * gcs : gnyq + 2 * gcn;
+                for (int t = threadIdx.x; t < N * P; t += kThreads) {
+                    const int c = t % P, j = t / P;
+                    const int iy = main ? y : col0 + c, iz = main ? col0 + c : NH;
+                    const double ky = kfull(iy, N, kval), kz = khalf(iz, kval);
+                    const cplx<R> z = r2[j * PP + c];
+                    d1[off(j, c)] = cplx<R>{(R)(ky * (double)z.im), (R)(-(ky * (double)z.re))};
+                    d2[off(j, c)] = cplx<R>{(R)(kz * (double)z.im), (R)(-(kz * (double)z.re))};
+                }
+            } else {
+                cplx<R>* d0 = main ? gspec : gnyq;
+                for (int t = threadIdx.x; t < N * P; t += kThreads) {
+                    const int c = t % P, j = t / P;
+                    d0[off(j, c)] = r2[j * PP + c];
+                }
             }
             __syncthreads();
         }
