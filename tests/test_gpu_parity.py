@@ -343,3 +343,30 @@ def test_deposit_shape_variants(PM3D, N, prec, monkeypatch):
     for v in ("1", "2", "3", "auto"):
         for u, w in zip(outs["0"], outs[v]):
             assert np.array_equal(u, w), (v, shapes[v])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_deferred_gather_is_invisible(PM3D, prec):
+    """A step leaves its forces as gradient meshes and the next step gathers them in its first
+    kernel.  Reading p after every step (gather + kick applied on their own), calling forces()
+    between steps (it replaces the gradient meshes, so the next step kicks with the stored
+    acceleration), or neither, must give the same trajectory bit for bit."""
+    from pbh_amd import ic
+    say("deferred gather", prec)
+    spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    outs = []
+    for mode in ("plain", "read_p", "forces"):
+        pm = PM3D(64, 1.0, 64, precision=prec)
+        pm.set_state(st["x"], st["p"])
+        a = st["a_i"]
+        for _ in range(4):
+            a, _dm = pm.step(a, 0.03 * a)
+            if mode == "read_p":
+                _ = pm.p
+            elif mode == "forces":
+                pm.forces(a)
+        outs.append((pm.x, pm.p))
+        del pm
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0]) and np.array_equal(outs[0][1], o[1])
