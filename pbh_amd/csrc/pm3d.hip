@@ -2,14 +2,17 @@
 //
 // One handle = one periodic box stepped on one HIP stream.  Device state is SoA in Lagrangian
 // particle order (particle i is lattice site i, pm3d.py:64-65).
-//   FP64 mode : x, p, acc as 3 x n doubles; deposit mesh 128-bit fixed point {lo,hi} per cell.
+//   FP64 mode : x, p, acc as 3 x n doubles; deposit mesh 128-bit fixed point {lo,hi} per cell at
+//               2^-52.
 //   mixed mode: cell (3 x n int32) + off (3 x n float), p and acc 3 x n float; deposit mesh one
-//               uint64 per cell at 2^-32; FFT, Green's function and gradient mesh in FP32.
+//               uint64 per cell at 2^-31; FFT, Green's function and gradient mesh in FP32.
 //
 // A step (pm3d.py:117-129) is:
-//   kick+drift -> deposit (+ far pass) -> r2c_z (fixed point -> delta fused) -> y fwd
-//   -> x (fwd FFT, Green's function, -ik, 3 inverse FFTs) -> y inv (3 comps) -> c2r_z (3 comps)
-//   -> gather (stores acc; the closing kick is applied with the next opening kick)
+//   gather of the previous step's force meshes + its closing kick + this step's opening kick and
+//   drift (one kernel) -> deposit (+ far pass) -> z and y forward per x plane (fixed point -> delta
+//   fused) -> x (forward FFT, Green's function, -ik, inverse FFTs) -> y and z inverse per x plane
+//   into the three force meshes, which the next step gathers.  Sizes whose half plane does not fit
+//   in LDS run the z, y, x, y, z passes as separate pencil kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
