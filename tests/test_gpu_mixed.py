@@ -103,3 +103,30 @@ def test_collapse_time_vs_oracle(precision):
     assert abs(r_gpu.t_c - r_ref.t_c) <= 1e-4 * r_ref.t_c
     if precision == "fp64":
         assert abs(r_gpu.t_c - r_ref.t_c) <= 1e-9 * r_ref.t_c
+
+
+def test_mixed_vs_fp64_at_256():
+    """The mixed bench workload at its own size (peak256: 256^3 mesh and particles, random field):
+    from one initial state (built on the device), three mixed steps agree with three FP64 steps
+    (the FP64 path is pinned to the oracle at the sizes the oracle runs in seconds) within the
+    mixed-precision bounds, and the fixed-point deposit conserves mass."""
+    from pbh_amd import ic
+    from pbh_amd.pm3d import PM3D
+    say("mixed vs fp64 at 256^3")
+    spec = ic.named("peak256")
+    pm64 = PM3D(spec.N, spec.L, spec.N, precision="fp64")
+    ic.device_ic(pm64, spec)
+    pmx = PM3D(spec.N, spec.L, spec.N, precision="mixed")
+    ic.device_ic(pmx, spec, engine=pm64)
+    a = spec.a_i
+    for _ in range(3):
+        a_next, d64 = pm64.step(a, 0.0125 * a)
+        _, dmx = pmx.step(a, 0.0125 * a)
+        a = a_next
+    r64, rmx = np.asarray(d64) + 1.0, np.asarray(dmx) + 1.0
+    say("256^3 mixed vs fp64: rho %.2e  x %.2e  p %.2e  mass %.1e" % (
+        rel(rmx, r64), pos_err(pmx.x, pm64.x, spec.L), rel(pmx.p, pm64.p), abs(rmx.mean() - 1.0)))
+    assert rel(rmx, r64) <= 1e-5
+    assert abs(rmx.mean() - 1.0) <= 1e-8 and abs(r64.mean() - 1.0) <= 1e-12
+    assert pos_err(pmx.x, pm64.x, spec.L) <= 1e-5
+    assert rel(pmx.p, pm64.p) <= 1e-5
