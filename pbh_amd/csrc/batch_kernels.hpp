@@ -38,7 +38,7 @@ struct MxBox {
     double* bmax;                // per-block maxima / minima
     unsigned long long* qslot;   // deposit maximum slot
     double kval;                 // 1/(N dx)
-    double cells_per_site;       // N / Np
+    double inv_c3;               // (Np / N)^3: 1 / (cells per lattice site)^3
     double qscale, nmean;
 };
 
@@ -95,7 +95,7 @@ __global__ __launch_bounds__((InvTile<float, N, Q>::NT)) void k_inv_plane_boxes(
 
 __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx_boxes(const MxBox* __restrict__ boxes) {
     const MxBox& b = boxes[blockIdx.y];
-    min_jacobian_mx_body(MxPos{b.cell, b.off, b.n}, b.Np, b.N, b.cells_per_site, b.bmax);
+    min_jacobian_mx_body(MxPos{b.cell, b.off, b.n}, b.Np, b.N, b.inv_c3, b.bmax);
 }
 
 __global__ __launch_bounds__(kThreads) void k_sched_advance_boxes(const MxBox* __restrict__ boxes, int nb) {

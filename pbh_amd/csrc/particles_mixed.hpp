@@ -306,38 +306,47 @@ __global__ __launch_bounds__(kThreads) void k_sched_advance(DevSched* __restrict
     sched_advance_body(sch, bmin, nb, jhist);
 }
 
-// min det(dx/dq) over the Lagrangian lattice from cell + offset (differences exact in cells).
-__device__ __forceinline__ void min_jacobian_mx_body(MxPos P, int Np, int N, double cells_per_site,
+// min det(dx/dq) over the Lagrangian lattice from cell + offset: the forward differences in cells
+// (exact integer cell differences, minimum image, plus the offset differences) form J*c with c the
+// cells per lattice site, so det(J) = det(J*c) * inv_c3, inv_c3 = 1/c^3 (one multiply instead of nine
+// divisions).  32-bit indices (3n < 2^31).
+__device__ __forceinline__ void min_jacobian_mx_body(MxPos P, int Np, int N, double inv_c3,
                                                      double* __restrict__ bmin) {
-    const long long n = P.n;
-    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const unsigned n = (unsigned)P.n;
+    const unsigned i = blockIdx.x * kThreads + threadIdx.x;
     double det = 1e300;
     if (i < n) {
-        const int ii = (int)i, a = ii / (Np * Np), b = (ii / Np) - a * Np, c = ii - (ii / Np) * Np;
-        const long long nb[3] = {((long long)(a + 1 == Np ? 0 : a + 1) * Np + b) * Np + c,
-                                 ((long long)a * Np + (b + 1 == Np ? 0 : b + 1)) * Np + c,
-                                 ((long long)a * Np + b) * Np + (c + 1 == Np ? 0 : c + 1)};
+        const unsigned np = (unsigned)Np, a = i / (np * np), r = i - a * (np * np), b = r / np, c = r - b * np;
+        const unsigned nb[3] = {((a + 1 == np ? 0u : a + 1) * np + b) * np + c, (a * np + (b + 1 == np ? 0u : b + 1)) * np + c,
+                                (a * np + b) * np + (c + 1 == np ? 0u : c + 1)};
+        int ci[3];
+        float fi[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            ci[d] = P.cell[d * n + i];
+            fi[d] = P.off[d * n + i];
+        }
         double J[3][3];
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                int dc = P.cell[d * n + nb[e]] - P.cell[d * n + i];
+                int dc = P.cell[d * n + nb[e]] - ci[d];
                 if (dc > N / 2) dc -= N;
                 if (dc < -N / 2) dc += N;
-                J[e][d] = ((double)dc + ((double)P.off[d * n + nb[e]] - (double)P.off[d * n + i])) / cells_per_site;
+                J[e][d] = (double)dc + ((double)P.off[d * n + nb[e]] - (double)fi[d]);
             }
         }
-        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        det = (J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) * inv_c3;
     }
     const double m = block_min(det);
     if (threadIdx.x == 0) bmin[blockIdx.x] = m;
 }
 
-__global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, int N, double cells_per_site,
+__global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, int N, double inv_c3,
                                                               double* __restrict__ bmin) {
-    min_jacobian_mx_body(P, Np, N, cells_per_site, bmin);
+    min_jacobian_mx_body(P, Np, N, inv_c3, bmin);
 }
 
 }  // namespace pm3d

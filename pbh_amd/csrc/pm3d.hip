@@ -729,6 +729,12 @@ int tune_deposit(Handle* h) {
     return 0;
 }
 
+// 1/c^3 for the mixed Jacobian, c = N/Np cells per lattice site
+double inv_c3(const Handle* h) {
+    const double c = (double)h->N / h->Np;
+    return 1.0 / (c * c * c);
+}
+
 // Gather blocks of kThreads consecutive particles: one x plane of the Lagrangian lattice is
 // Np^2/kThreads blocks; an XCD keeps 1/8 of every plane, so its corner rows stay in its L2.
 XMap gather_map(Handle* h) {
@@ -939,7 +945,7 @@ int launch_sched_step(Handle* h) {
     if (!h->mixed)
         k_min_jacobian<<<g, kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, h->bmax);
     else
-        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, (double)h->N / h->Np, h->bmax);
+        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, inv_c3(h), h->bmax);
     k_sched_advance<<<1, kThreads, 0, h->st>>>(h->sched, h->bmax, (int)g, h->jhist);
     return check_launch(h, "scheduled step");
 }
@@ -989,7 +995,7 @@ MxBox box_of(Handle* h) {
     b.bmax = h->bmax;
     b.qslot = qmax_slot(h);
     b.kval = 1.0 / ((double)h->N * h->dx);
-    b.cells_per_site = (double)h->N / h->Np;
+    b.inv_c3 = inv_c3(h);
     b.qscale = h->qscale;
     b.nmean = h->nmean;
     (void)specN;
@@ -1495,7 +1501,7 @@ int pm3d_min_jacobian(void* handle, double* jmin) {
     if (!h->mixed)
         k_min_jacobian<<<g, kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, h->bmax);
     else
-        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, (double)h->N / h->Np, h->bmax);
+        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, inv_c3(h), h->bmax);
     std::vector<double> hb(g);
     HCHK(h, hipMemcpyAsync(hb.data(), h->bmax, g * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
