@@ -350,13 +350,14 @@ def test_deferred_gather_is_invisible(PM3D, prec):
     """A step leaves its forces as gradient meshes and the next step gathers them in its first
     kernel.  Reading p after every step (gather + kick applied on their own), calling forces()
     between steps (it replaces the gradient meshes, so the next step kicks with the stored
-    acceleration), or neither, must give the same trajectory bit for bit."""
+    acceleration), assigning x (FP64: an exact round trip; the acceleration of the old positions is
+    kept, as the reference keeps _acc), or none of these, must give the same trajectory bit for bit."""
     from pbh_amd import ic
     say("deferred gather", prec)
     spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
     st = ic.make(spec)
     outs = []
-    for mode in ("plain", "read_p", "forces"):
+    for mode in ("plain", "read_p", "forces") + (("set_x",) if prec == "fp64" else ()):
         pm = PM3D(64, 1.0, 64, precision=prec)
         pm.set_state(st["x"], st["p"])
         a = st["a_i"]
@@ -366,6 +367,8 @@ def test_deferred_gather_is_invisible(PM3D, prec):
                 _ = pm.p
             elif mode == "forces":
                 pm.forces(a)
+            elif mode == "set_x":
+                pm.x = pm.x
         outs.append((pm.x, pm.p))
         del pm
     for o in outs[1:]:
