@@ -2,7 +2,7 @@
 deposit, FP64 scalars) against the FP64 oracle, and the first-shell-crossing time against the
 oracle's.  Every call goes through the C-ABI of _pm3d_hip.so.
 
-Tolerances: density max|drho|/max|rho| <= 1e-5 (north-star mixed-precision bound); positions
+Tolerances: density max|drho|/max|rho| <= 1e-5 (the mixed-precision density bound); positions
 (minimum image, / L) and momenta <= 1e-5 after 20 steps; collapse time |dt_c| <= 1e-4 t_c.
 """
 import sys
