@@ -1002,18 +1002,31 @@ MxBox box_of(Handle* h) {
     return b;
 }
 
-// One scheduled step of nb boxes of mesh size N in eight launches (the kernels and grids of
-// launch_sched_step per box, with the boxes along the grid's y dimension).
-template <int N>
-void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
-    using S = ColShape<N, false>;
-    const unsigned g = grid_for(h0->n);
-    k_gather_kick_drift_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev, gather_map(h0));
+// Column deposit and far pass of nb boxes of mesh size N in column shape variant V.
+template <int N, int V>
+void launch_boxes_deposit(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
+    using S = ColShape<N, false, V>;
     k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, XMap{1, 1});
     if constexpr (S::NBLK <= kFarColsMax)
         k_deposit_far_bal_boxes<<<dim3(far_bal_grid(h0->n), nb), kThreads, 0, st>>>(dev, S::NBLK, S::BX * S::BY * N);
     else
         k_deposit_far_seg_boxes<<<dim3(S::NBLK, kFarSub, nb), kThreads, 0, st>>>(dev, S::BX * S::BY * N);
+}
+
+// One scheduled step of nb boxes of mesh size N in eight launches (the kernels and grids of
+// launch_sched_step per box, with the boxes along the grid's y dimension).  The deposit's column
+// shape and the x pass's tile variant are the ones the tuner chose for this mesh size (every member
+// of the class shares them).
+template <int N>
+void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
+    const unsigned g = grid_for(h0->n);
+    k_gather_kick_drift_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev, gather_map(h0));
+    switch (h0->depv) {
+        case 1: launch_boxes_deposit<N, 1>(h0, dev, nb, st); break;
+        case 2: launch_boxes_deposit<N, 2>(h0, dev, nb, st); break;
+        case 3: launch_boxes_deposit<N, 3>(h0, dev, nb, st); break;
+        default: launch_boxes_deposit<N, 0>(h0, dev, nb, st); break;
+    }
     k_fwd_plane_boxes<N><<<dim3(2 * N, nb), PlaneTile<float, N>::NT, 0, st>>>(dev, xmap(2 * N, 2));
     switch (h0->fftv[FP_X]) {
 #define PM3D_XBOX(V)                                                                                    \
