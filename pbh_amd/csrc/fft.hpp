@@ -235,6 +235,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
     __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
     double lmax = 0.0;
+    const double inv_nmean = 1.0 / nmean;  // rho*(1/n_mean) - 1: exact for a power-of-two n_mean
     for (int t = threadIdx.x; t < P * NC; t += kThreads) {
         const int c = t / NC, m = t - c * NC;
         const size_t g = (row0 + c) * N + 2 * m;
@@ -243,8 +244,8 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
             const unsigned long long* mesh = (const unsigned long long*)in;
             const double r0 = fx_value<WIDE>(mesh, g, qscale), r1 = fx_value<WIDE>(mesh, g + 1, qscale);
             lmax = fmax(lmax, fmax(r0, r1));
-            v0 = (R)(r0 / nmean - 1.0);
-            v1 = (R)(r1 / nmean - 1.0);
+            v0 = (R)(r0 * inv_nmean - 1.0);
+            v1 = (R)(r1 * inv_nmean - 1.0);
         } else {
             v0 = ((const R*)in)[g];
             v1 = ((const R*)in)[g + 1];

@@ -189,6 +189,9 @@ __device__ __forceinline__ void fwd_plane_body(const void* __restrict__ in, cplx
     const unsigned lin = xcd_map(blockIdx.x, gridDim.x, xm);
     const int x = (int)(lin >> 1), h = (int)(lin & 1);
     double lmax = 0.0;
+    // delta = rho/n_mean - 1 as rho*(1/n_mean) - 1: the same bits when n_mean is a power of two (1
+    // for a particle per cell), within an ulp otherwise, without a division per cell
+    const double inv_nmean = 1.0 / nmean;
     // combined rows r[y'] = row(y') +/- row(y'+H), packed as (r[2m], r[2m+1]) at T[m*S + y'];
     // each thread issues the loads of LB cell pairs before it converts and stores any of them
     constexpr int IT = (H * H + NT - 1) / NT, LB = LD::B;
@@ -215,10 +218,10 @@ __device__ __forceinline__ void fwd_plane_body(const void* __restrict__ in, cplx
                     LD::values(ra[b], qscale, r[0], r[1]);
                     LD::values(rb[b], qscale, r[2], r[3]);
                     lmax = fmax(lmax, fmax(fmax(r[0], r[1]), fmax(r[2], r[3])));
-                    a0 = (R)(r[0] / nmean - 1.0);
-                    a1 = (R)(r[1] / nmean - 1.0);
-                    b0 = (R)(r[2] / nmean - 1.0);
-                    b1 = (R)(r[3] / nmean - 1.0);
+                    a0 = (R)(r[0] * inv_nmean - 1.0);
+                    a1 = (R)(r[1] * inv_nmean - 1.0);
+                    b0 = (R)(r[2] * inv_nmean - 1.0);
+                    b1 = (R)(r[3] * inv_nmean - 1.0);
                 } else {
                     a0 = ra[b].v[0];
                     a1 = ra[b].v[1];
