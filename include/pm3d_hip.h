@@ -183,7 +183,8 @@ int pm3d_copy_state(void* dst_handle, void* src_handle);
  * device keeps a step counter; every scheduled step reads entry [counter], records the minimum of
  * det(dx/dq) of its new positions in a history and advances the counter.  A schedule continues a
  * stepped state (the force meshes of the current positions and a pending closing kick must exist:
- * run one pm3d_step first).  Steps past the end of the table leave the state unchanged. */
+ * run one pm3d_step first).  A step whose Jacobian minimum is <= 0 (the first shell crossing) ends
+ * the table there; steps past the end of the table leave the state unchanged. */
 typedef struct pm3d_sched_entry {
     double c1p, denp;  /* closing kick of the previous step: p += (c1p*acc)/denp */
     double c1, den1;   /* opening kick */

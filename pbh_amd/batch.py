@@ -86,8 +86,10 @@ def _finish(m: Member, crossed_at: int | None):
                               m.jhist[crossed_at], list(zip(m.a_vals[: crossed_at + 1], m.jhist[: crossed_at + 1])))
 
 
-def run_batched(members: list[Member], max_steps: int = 4000, poll: int = 8, batch: Batch | None = None):
-    """Step every member to its first shell crossing (or a_max) in one graph-replayed batch."""
+def run_batched(members: list[Member], max_steps: int = 4000, poll: int = 32, batch: Batch | None = None):
+    """Step every member to its first shell crossing (or a_max) in one graph-replayed batch.  A member's
+    schedule ends on the device at its crossing (its later steps do nothing), so `poll` replays run
+    between host status reads without wasted work."""
     batch = batch or Batch()
     t0 = time.perf_counter()
     active = []
@@ -108,6 +110,7 @@ def run_batched(members: list[Member], max_steps: int = 4000, poll: int = 8, bat
         m._sched_a = avals
         active.append(m)
     batch.set([m.pm for m in active])
+    built = len(active)  # members in the captured graph
     while active:
         batch.run(poll)
         batch.synchronize()
@@ -125,8 +128,11 @@ def run_batched(members: list[Member], max_steps: int = 4000, poll: int = 8, bat
                 m.t_done = time.perf_counter() - t0
             else:
                 still.append(m)
-        if len(still) != len(active):
+        # a finished member's steps do nothing on the device (its schedule ends at the crossing), so
+        # the graph is rebuilt without them only once a quarter of its members have finished
+        if still and built - len(still) >= max(1, built // 4):
             batch.set([m.pm for m in still])
+            built = len(still)
         active = still
     for m in members:
         if getattr(m, "_sched_a", None) is not None:

@@ -32,7 +32,7 @@ def _members(seeds, precision="mixed", da_frac=0.02, engines=None):
     return out, engines
 
 
-def batched_vs_plain(seeds=range(100, 164), precision="mixed", poll=8):
+def batched_vs_plain(seeds=range(100, 164), precision="mixed", poll=32):
     seeds = list(seeds)
     engines = {}
     plain, engines = _members(seeds, precision, engines=engines)
@@ -69,5 +69,8 @@ def batched_vs_plain(seeds=range(100, 164), precision="mixed", poll=8):
 
 
 if __name__ == "__main__":
+    import argparse
     import json
-    print(json.dumps(batched_vs_plain()))
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--poll", type=int, default=32, help="graph replays between host status reads")
+    print(json.dumps(batched_vs_plain(poll=ap.parse_args().poll)))

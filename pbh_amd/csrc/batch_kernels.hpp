@@ -42,6 +42,10 @@ struct MxBox {
     double qscale, nmean;
 };
 
+// A member whose schedule has ended (table done, or ended at its first shell crossing) skips the
+// step: every kernel of a replay sees the same counter (it changes only in the step's last kernel).
+__device__ __forceinline__ bool box_done(const MxBox& b) { return b.sched->counter >= b.sched->nsteps; }
+
 __device__ __forceinline__ SrcMx box_src(const MxBox& b) {
     return SrcMx{b.cell, b.cell + b.n, b.cell + 2 * b.n, b.off, b.off + b.n, b.off + 2 * b.n, b.N};
 }
@@ -49,28 +53,33 @@ __device__ __forceinline__ SrcMx box_src(const MxBox& b) {
 // The previous step's gather fused with this step's kicks and drift (gather_kick_drift_mx_body).
 __global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
+    if (box_done(b)) return;
     gather_kick_drift_mx_body(b.cell, b.off, b.p, b.n, b.N, b.grad, 1, 0.f, 0.f, 0.f, b.dscal + 1, b.sched, b.tab, xm);
 }
 
 template <int BX, int BY, int N>
 __global__ __launch_bounds__(1024) void k_deposit_col_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
+    if (box_done(b)) return;
     deposit_col_body<SrcMx, BX, BY, N, false, 1024>(box_src(b), b.mesh, b.far, b.farcnt, xm);
 }
 
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox* __restrict__ boxes, int seg) {
     const MxBox& b = boxes[blockIdx.z];
+    if (box_done(b)) return;
     deposit_far_seg_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, seg, b.N);
 }
 
 __global__ __launch_bounds__(kThreads) void k_deposit_far_bal_boxes(const MxBox* __restrict__ boxes, int ncol, int seg) {
     const MxBox& b = boxes[blockIdx.y];
+    if (box_done(b)) return;
     deposit_far_bal_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, ncol, seg, b.N, blockIdx.x, gridDim.x);
 }
 
 template <int N>
 __global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
+    if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2);
     fwd_plane_body<float, N, true, false>(b.mesh, b.spec, b.spec + specN, b.tw, b.bmax, b.dscal, b.qscale, b.nmean, xm);
 }
@@ -80,6 +89,7 @@ __global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(c
 template <int N, int V>
 __global__ __launch_bounds__(kThreads) void k_x_poisson_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
+    if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
     x_pass_body<float, N, 2, V>(b.spec, b.spec + specN, b.gspec, b.gspec + specN, tot, tot, b.tw, b.kval, 0.0, 0.0, 0,
                                 b.sched, b.tab, b.bmax, 2 * N, b.qslot, xm);
@@ -88,6 +98,7 @@ __global__ __launch_bounds__(kThreads) void k_x_poisson_boxes(const MxBox* __res
 template <int N, int Q>
 __global__ __launch_bounds__((InvTile<float, N, Q>::NT)) void k_inv_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
+    if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
     inv_plane_body<float, N, Q>(b.gspec, b.gspec + specN, tot, tot, b.grad, (size_t)N * N * N, b.tw,
                                 (float)(1.0 / ((double)N * N * N)), xm);
@@ -95,6 +106,7 @@ __global__ __launch_bounds__((InvTile<float, N, Q>::NT)) void k_inv_plane_boxes(
 
 __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx_boxes(const MxBox* __restrict__ boxes) {
     const MxBox& b = boxes[blockIdx.y];
+    if (box_done(b)) return;
     min_jacobian_mx_body(MxPos{b.cell, b.off, b.n}, b.Np, b.N, b.inv_c3, b.bmax);
 }
 

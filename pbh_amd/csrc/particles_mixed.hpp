@@ -286,7 +286,8 @@ __global__ __launch_bounds__(kThreads) void k_f32_soa_to_f64(const float* __rest
 }
 
 // Last kernel of a scheduled step: reduce the Jacobian kernel's per-block minima into
-// jhist[counter] and advance the schedule.
+// jhist[counter] and advance the schedule.  A minimum <= 0 (the first shell crossing) ends the
+// table there (nsteps = counter), so the member's later steps do nothing.
 __device__ __forceinline__ void sched_advance_body(DevSched* __restrict__ sch, const double* __restrict__ bmin,
                                                    int nb, double* __restrict__ jhist) {
     double m = 1e300;
@@ -297,6 +298,7 @@ __device__ __forceinline__ void sched_advance_body(DevSched* __restrict__ sch, c
         if (k < sch->nsteps) {
             jhist[k] = m;
             sch->counter = k + 1;
+            if (m <= 0.0) sch->nsteps = k + 1;
         }
     }
 }
