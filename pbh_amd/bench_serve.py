@@ -67,9 +67,10 @@ if __name__ == "__main__":
     ap.add_argument("--n", type=int, default=16)
     ap.add_argument("--plain", type=int, default=4)
     ap.add_argument("--chains", action="store_true")
+    ap.add_argument("--poll", type=int, default=8, help="graph replays between admissions and status reads")
     args = ap.parse_args()
     rates = [float(x) for x in args.rates.split(",")]
-    res = {"N": args.N, "sweep": sweep(args.N, rates, args.n)}
+    res = {"N": args.N, "poll": args.poll, "sweep": sweep(args.N, rates, args.n, poll=args.poll)}
     if args.plain:
         res["plain_fp64"] = plain_fp64(config_requests(args.plain, N=args.N, seed0=1000))
         print(json.dumps(res["plain_fp64"]), flush=True)
