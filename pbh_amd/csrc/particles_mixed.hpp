@@ -56,18 +56,6 @@ __device__ __forceinline__ void mx_scatter(const int lo[3], const float f[3], in
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_deposit_far_mx(MxPos P, int N, unsigned long long* __restrict__ mesh,
-                                                             const int* __restrict__ far,
-                                                             const unsigned int* __restrict__ nfar) {
-    const unsigned int m = *nfar;
-    for (unsigned int t = blockIdx.x * kThreads + threadIdx.x; t < m; t += gridDim.x * kThreads) {
-        int lo[3];
-        float f[3];
-        P.load(far[t], lo, f);
-        mx_scatter(lo, f, N, mesh);
-    }
-}
-
 __global__ __launch_bounds__(kThreads) void k_deposit_atomic_mx(MxPos P, int N, unsigned long long* __restrict__ mesh) {
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     if (i >= P.n) return;
