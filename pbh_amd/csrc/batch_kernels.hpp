@@ -24,7 +24,7 @@ struct MxBox {
     float* acc;
     long long n;
     int N, Np;
-    unsigned long long* dscal;  // [0] far counter, [1] sticky bad count
+    unsigned long long* dscal;  // [1] sticky bad count
     DevSched* sched;
     const DevStep* tab;
     double* jhist;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(c
     const MxBox& b = boxes[blockIdx.y];
     if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2);
-    fwd_plane_body<float, N, true, false>(b.mesh, b.spec, b.spec + specN, b.tw, b.bmax, b.dscal, b.qscale, b.nmean, xm);
+    fwd_plane_body<float, N, true, false>(b.mesh, b.spec, b.spec + specN, b.tw, b.bmax, b.qscale, b.nmean, xm);
 }
 
 // The Poisson x pass with the scheduled Green's function factor; its last block reduces the

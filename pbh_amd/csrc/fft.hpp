@@ -224,13 +224,11 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R
 // Rows (x,y) of the real mesh -> half spectrum along kz.  FIXED: the input is the fixed-point
 // deposit (WIDE: 128-bit cells), converted on the fly to delta = rho/n_mean - 1 (pm3d.py:103-104),
 // with each block's maximum rho stored to bmax[blockIdx.x] (reduced by the following y pass, so no
-// two blocks contend for one address).  Block 0 also clears the deposit's far-particle counter for
-// the next deposit.
+// two blocks contend for one address).
 template <typename R, int N, bool FIXED, bool WIDE, int V>
 __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                     cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
-                                                    double* __restrict__ bmax, unsigned long long* __restrict__ nfar,
-                                                    double qscale, double nmean) {
+                                                    double* __restrict__ bmax, double qscale, double nmean) {
     constexpr int NC = N / 2, P = Plan<R, N, V>::PZ, PP = lds_pad<R>(P);
     __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;
@@ -254,10 +252,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
     }
     if constexpr (FIXED) {
         const double m = block_max_nonneg(lmax);
-        if (threadIdx.x == 0) {
-            bmax[blockIdx.x] = m;
-            if (blockIdx.x == 0 && nfar) *nfar = 0ull;
-        }
+        if (threadIdx.x == 0) bmax[blockIdx.x] = m;
     }
     __syncthreads();
     const cplx<R>* Z = lds_fft<R, NC, P, false, 2, radix_l2<R, PK_R2C>(), Plan<R, N, V>::INPLACE>(A, B, tw);

@@ -175,12 +175,12 @@ struct PlaneLoadBatch<R, false, false> {
 // ---------------------------------------------------------------------------- forward, plane x
 // Workgroup (x, h), logical id 2x + h.  FIXED: the input is the fixed-point deposit (WIDE: 128-bit
 // cells), converted to delta = rho/n_mean - 1 on load (pm3d.py:103-104); both workgroups of a plane
-// store the plane's maximum rho to bmax[2x + h] (reduced by the x pass).  Block 0 clears *nfar.
+// store the plane's maximum rho to bmax[2x + h] (reduced by the x pass).
 template <typename R, int N, bool FIXED, bool WIDE>
 __device__ __forceinline__ void fwd_plane_body(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
                                                double* __restrict__ bmax,
-                                               unsigned long long* __restrict__ nfar, double qscale,
+                                               double qscale,
                                                double nmean, XMap xm) {
     using PT = PlaneTile<R, N>;
     using LD = PlaneLoadBatch<R, FIXED, WIDE>;
@@ -234,10 +234,7 @@ __device__ __forceinline__ void fwd_plane_body(const void* __restrict__ in, cplx
     }
     if constexpr (FIXED) {
         const double mx = block_max_nonneg_nt<NT>(lmax);
-        if (threadIdx.x == 0) {
-            bmax[lin] = mx;
-            if (lin == 0 && nfar) *nfar = 0ull;
-        }
+        if (threadIdx.x == 0) bmax[lin] = mx;
     }
     __syncthreads();
     // z: H-point transform of every combined row (pencil y', element m)
@@ -275,9 +272,9 @@ template <typename R, int N, bool FIXED, bool WIDE>
 __global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_fwd_plane(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                                      cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
                                                                      double* __restrict__ bmax,
-                                                                     unsigned long long* __restrict__ nfar, double qscale,
+                                                                     double qscale,
                                                                      double nmean, XMap xm) {
-    fwd_plane_body<R, N, FIXED, WIDE>(in, spec, nyq, tw, bmax, nfar, qscale, nmean, xm);
+    fwd_plane_body<R, N, FIXED, WIDE>(in, spec, nyq, tw, bmax, qscale, nmean, xm);
 }
 
 // ---------------------------------------------------------------------------- inverse, plane x

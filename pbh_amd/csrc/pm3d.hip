@@ -84,7 +84,7 @@ struct Handle {
     void* gspec = nullptr;  // 3 x the same
     void* grad = nullptr;   // [3][M] R (component-major)
     void* tw = nullptr;     // exp(-2 pi i k/N), complex R
-    // device scalars: [0] nfar, [1] bad count (sticky until pm3d_synchronize), [2..7] unused,
+    // device scalars: [0] unused, [1] bad count (sticky until pm3d_synchronize), [2..7] unused,
     // [8 + s % kRing] max rho (double bits) of deposit serial s
     unsigned long long* dscal = nullptr;
     long long serial = 0;
@@ -220,14 +220,13 @@ struct Fft {
         const XMap xm = xmap(g, 2);  // the two workgroups of a plane read the same rows: one XCD
         Timed t(h, KC_FWD_ZY);
         if (fixed && h->wide)
-            k_fwd_plane<R, N, true, true><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal, h->qscale,
+            k_fwd_plane<R, N, true, true><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->qscale,
                                                               h->nmean, xm);
         else if (fixed)
-            k_fwd_plane<R, N, true, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal, h->qscale,
+            k_fwd_plane<R, N, true, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->qscale,
                                                                h->nmean, xm);
         else
-            k_fwd_plane<R, N, false, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, nullptr, 0.0, 1.0,
-                                                                xm);
+            k_fwd_plane<R, N, false, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, 0.0, 1.0, xm);
         h->bmax_pending = fixed ? (int)g : 0;
     }
 
@@ -263,14 +262,13 @@ struct Fft {
         const unsigned gz = r2c_grid<V>();
         Timed t(h, KC_R2C_Z);
         if (fixed && h->wide)
-            k_r2c_z<R, N, true, true, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal,
+            k_r2c_z<R, N, true, true, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax,
                                                                     h->qscale, h->nmean);
         else if (fixed)
-            k_r2c_z<R, N, true, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->dscal,
+            k_r2c_z<R, N, true, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax,
                                                                      h->qscale, h->nmean);
         else
-            k_r2c_z<R, N, false, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, nullptr,
-                                                                      0.0, 1.0);
+            k_r2c_z<R, N, false, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, 0.0, 1.0);
     }
 
     // nbmax: number of per-block maxima the r2c launch left in h->bmax (0 when the input was real)
