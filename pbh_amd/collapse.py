@@ -49,16 +49,3 @@ def run_to_crossing(pm, a_start: float, da_frac: float, a_max: float, max_steps:
         a, j_prev = a_new, j
     return CollapseResult(False, float("nan"), float("nan"), k, a, j_prev, hist)
 
-
-class OracleStepper:
-    """Adapter giving the CPU oracle the same step/min_jacobian interface (tests and baselines)."""
-
-    def __init__(self, oracle_pm, min_jacobian_fn):
-        self.pm = oracle_pm
-        self._mj = min_jacobian_fn
-
-    def step(self, a, da):
-        return self.pm.step(a, da)
-
-    def min_jacobian(self):
-        return self._mj(self.pm.x, self.pm.Np, self.pm.L)

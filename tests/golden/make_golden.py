@@ -7,7 +7,7 @@ vectors for the functions that closed forms and invariants do not pin on their o
 
 Output: tests/golden/pm3d_golden.npz (data only: inputs and expected outputs) and
 tests/golden/pm3d_golden_meta.json.  Nothing here is imported by the product or the GPU tests' code
-paths; the GPU box never sees /root/reference.
+paths; the GPU tests never read /root/reference.
 
     python tests/golden/make_golden.py
 """
