@@ -14,6 +14,7 @@
 //   into the three force meshes, which the next step gathers.  Sizes whose half plane does not fit
 //   in LDS run the z, y, x, y, z passes as separate pencil kernels.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <array>
@@ -151,16 +152,20 @@ hipEvent_t take_event(Handle* h) {
     return e;
 }
 
-// Brackets one launch with events when timing is on.
+// Times the launches of one scope when timing is on.  A scope of one launch made through
+// `launch` (`one` = true) has the dispatch itself stamp the event pair (hipExtLaunchKernelGGL: the
+// kernel's own start and end, the duration a profiler's kernel trace reports); any other scope is
+// bracketed by two event records, whose time includes the gaps around its launches.
 struct Timed {
     Handle* h;
-    hipEvent_t e1 = nullptr;
-    Timed(Handle* h_, int kc) : h(h_) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool one;
+    Timed(Handle* h_, int kc, bool one_launch = false) : h(h_), one(one_launch) {
         if (h->timing) {
-            hipEvent_t e0 = take_event(h);
+            e0 = take_event(h);
             e1 = take_event(h);
             if (e0 && e1) {
-                hipEventRecord(e0, h->st);
+                if (!one) hipEventRecord(e0, h->st);
                 h->ev_used.push_back({kc, {e0, e1}});
             } else {
                 e1 = nullptr;
@@ -168,7 +173,16 @@ struct Timed {
         }
     }
     ~Timed() {
-        if (e1) hipEventRecord(e1, h->st);
+        if (e1 && !one) hipEventRecord(e1, h->st);
+    }
+    // kernel<<<g, b, 0, h->st>>>(args...), every argument converted to the kernel's parameter type
+    template <typename... P, typename... A>
+    void launch(void (*kernel)(P...), dim3 g, dim3 b, A... args) {
+        static_assert(sizeof...(P) == sizeof...(A), "argument count");
+        if (e1 && one)
+            hipExtLaunchKernelGGL(kernel, g, b, 0, h->st, e0, e1, 0, static_cast<P>(args)...);
+        else
+            kernel<<<g, b, 0, h->st>>>(static_cast<P>(args)...);
     }
 };
 
@@ -218,15 +232,13 @@ struct Fft {
         const cplx<R>* tw = (const cplx<R>*)h->tw;
         const unsigned g = 2 * N;
         const XMap xm = xmap(g, 2);  // the two workgroups of a plane read the same rows: one XCD
-        Timed t(h, KC_FWD_ZY);
+        Timed t(h, KC_FWD_ZY, true);
         if (fixed && h->wide)
-            k_fwd_plane<R, N, true, true><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->qscale,
-                                                              h->nmean, xm);
+            t.launch(k_fwd_plane<R, N, true, true>, g, NT, in, spec, spec + specN, tw, h->bmax, h->qscale, h->nmean, xm);
         else if (fixed)
-            k_fwd_plane<R, N, true, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax, h->qscale,
-                                                               h->nmean, xm);
+            t.launch(k_fwd_plane<R, N, true, false>, g, NT, in, spec, spec + specN, tw, h->bmax, h->qscale, h->nmean, xm);
         else
-            k_fwd_plane<R, N, false, false><<<g, NT, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, 0.0, 1.0, xm);
+            t.launch(k_fwd_plane<R, N, false, false>, g, NT, in, spec, spec + specN, tw, nullptr, 0.0, 1.0, xm);
         h->bmax_pending = fixed ? (int)g : 0;
     }
 
@@ -235,9 +247,9 @@ struct Fft {
         constexpr int NT = InvTile<R, N, Q>::NT;
         const R scale = (R)(1.0 / ((double)N * N * N));
         const unsigned g = Q * N * ncomp;
-        Timed t(h, KC_INV_YZ);
-        k_inv_plane<R, N, Q><<<g, NT, 0, h->st>>>(base, base + specN, tot, tot, (R*)out, (size_t)N * N * N,
-                                                  (const cplx<R>*)h->tw, scale, xmap(g, Q));
+        Timed t(h, KC_INV_YZ, true);
+        t.launch(k_inv_plane<R, N, Q>, g, NT, base, base + specN, tot, tot, (R*)out, (size_t)N * N * N,
+                 (const cplx<R>*)h->tw, scale, xmap(g, Q));
     }
     // split 2 ways along y: measured on MI355X against 4 and 8 (smaller tiles, more workgroups per
     // CU, the plane read 4 or 8 times from L2), 128^3 FP64 / 256^3 FP32 inverse pass: 30.2 / 159 us
@@ -292,7 +304,7 @@ struct Fft {
         constexpr int P = Plan<R, N, V>::PX;
         const unsigned g = N * (N / 2 / P) + N / P;
         const double kval = 1.0 / ((double)N * h->dx);
-        Timed t(h, KC_X_POISSON);
+        Timed t(h, KC_X_POISSON, true);
         const DevSched* sch = h->use_sched ? h->sched : nullptr;
         const XMap xm = xmap(g, pencil_share<P>());
         // the plane forward pass leaves the deposit maximum in per-block pieces: reduced here
@@ -300,17 +312,17 @@ struct Fft {
         unsigned long long* slot = nb > 0 ? qmax_slot(h) : nullptr;
         h->bmax_pending = 0;
         if (mode == 0)
-            k_x_pass<R, N, 0, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0,
-                                                            0, nullptr, nullptr, h->bmax, nb, slot, xm);
+            t.launch(k_x_pass<R, N, 0, V>, g, kThreads, spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0,
+                     nullptr, nullptr, h->bmax, nb, slot, xm);
         else if (mode == 1)
-            k_x_pass<R, N, 1, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0,
-                                                            0, nullptr, nullptr, h->bmax, nb, slot, xm);
+            t.launch(k_x_pass<R, N, 1, V>, g, kThreads, spec, spec + specN, nullptr, nullptr, 0, 0, tw, kval, 0, 0, 0,
+                     nullptr, nullptr, h->bmax, nb, slot, xm);
         else if (mode == 2)
-            k_x_pass<R, N, 2, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s,
-                                                            sm2, sm2 > 0.0 ? 1 : 0, sch, h->tab, h->bmax, nb, slot, xm);
+            t.launch(k_x_pass<R, N, 2, V>, g, kThreads, spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, s, sm2,
+                     sm2 > 0.0 ? 1 : 0, sch, h->tab, h->bmax, nb, slot, xm);
         else
-            k_x_pass<R, N, 3, V><<<g, kThreads, 0, h->st>>>(spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, 0,
-                                                            0, 0, nullptr, nullptr, h->bmax, nb, slot, xm);
+            t.launch(k_x_pass<R, N, 3, V>, g, kThreads, spec, spec + specN, gs, gs + specN, tot, tot, tw, kval, 0, 0, 0,
+                     nullptr, nullptr, h->bmax, nb, slot, xm);
     }
 
     template <int V>
@@ -541,19 +553,19 @@ template <int N, bool WIDE, int V, class Src>
 void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh) {
     using S = ColShape<N, WIDE, V>;
     {
-        Timed t(h, KC_DEPOSIT);
-        k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024><<<S::NBLK, 1024, 0, h->st>>>(
-            src, mesh, h->far, h->farcnt, XMap{1, 1});
+        Timed t(h, KC_DEPOSIT, true);
+        t.launch(k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024>, S::NBLK, 1024, src, mesh, h->far, h->farcnt,
+                 XMap{1, 1});
     }
-    Timed t(h, KC_DEPOSIT_FAR);
+    Timed t(h, KC_DEPOSIT_FAR, true);
     if constexpr (S::NBLK <= kFarColsMax) {
-        k_deposit_far_bal<Src, WIDE><<<far_bal_grid(h->n), kThreads, 0, h->st>>>(src, mesh, h->far, h->farcnt, S::NBLK,
-                                                                                S::BX * S::BY * N, N);
+        t.launch(k_deposit_far_bal<Src, WIDE>, far_bal_grid(h->n), kThreads, src, mesh, h->far, h->farcnt, S::NBLK,
+                 S::BX * S::BY * N, N);
     } else {
         // a column lists at most its own BX*BY*N sites: kFarSub workgroups per column bound the work
         // of the fullest one
-        k_deposit_far_seg<Src, WIDE><<<dim3(S::NBLK, kFarSub), kThreads, 0, h->st>>>(src, mesh, h->far, h->farcnt,
-                                                                                  S::BX * S::BY * N, N);
+        t.launch(k_deposit_far_seg<Src, WIDE>, dim3(S::NBLK, kFarSub), kThreads, src, mesh, h->far, h->farcnt,
+                 S::BX * S::BY * N, N);
     }
 }
 
@@ -1440,16 +1452,16 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
         // gather of the force meshes (pm3d.py:113, the previous step's or the solve above) +
         // pending closing kick of the previous step (:127) + opening kick (:122) + drift
         // (:123-124) in one pass; FP64 keeps the reference's two separate roundings of p
-        Timed t(h, KC_GATHER_KICK);
+        Timed t(h, KC_GATHER_KICK, true);
         if (!h->mixed)
-            k_gather_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(
-                h->x, h->p, h->n, h->N, h->dx, (const double*)h->grad, h->kick_pending ? 1 : 0, h->pend_c1, h->pend_den,
-                sc->c1, sc->den1, sc->da, sc->den_d, h->L, h->dscal + 1, nullptr, nullptr, gather_map(h));
+            t.launch(k_gather_kick_drift, grid_for(h->n), kThreads, h->x, h->p, h->n, h->N, h->dx,
+                     (const double*)h->grad, h->kick_pending ? 1 : 0, h->pend_c1, h->pend_den, sc->c1, sc->den1, sc->da,
+                     sc->den_d, h->L, h->dscal + 1, nullptr, nullptr, gather_map(h));
         else
-            k_gather_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(
-                h->cell, h->off, h->pf, h->n, h->N, (const float*)h->grad, h->kick_pending ? 1 : 0,
-                (float)(h->kick_pending ? h->pend_c1 / h->pend_den : 0.0), (float)(sc->c1 / sc->den1),
-                (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr, gather_map(h));
+            t.launch(k_gather_kick_drift_mx, grid_for(h->n), kThreads, h->cell, h->off, h->pf, h->n, h->N,
+                     (const float*)h->grad, h->kick_pending ? 1 : 0,
+                     (float)(h->kick_pending ? h->pend_c1 / h->pend_den : 0.0), (float)(sc->c1 / sc->den1),
+                     (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr, gather_map(h));
     } else {
         // the stored acceleration (a state whose force meshes were replaced after it was gathered)
         Timed t(h, KC_KICK_DRIFT);
