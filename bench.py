@@ -11,7 +11,7 @@ the K timed steps runs one gather (the last warm-up step's at the start of the f
 
 Prints one JSON line: value = particle-steps/s = n*K/wall over K timed steps; `roofline` for the
 kernel with the largest share of step time (algorithmic bytes per launch / average launch time
-from hipEvents on the library's stream); `cpu_baseline` = the CPU FP64 oracle on the same workload,
+from the start/stop hipEvents that each launch's dispatch stamps on the library's stream); `cpu_baseline` = the CPU FP64 oracle on the same workload,
 3 steps, single-threaded numpy (1 core).
 """
 from __future__ import annotations
@@ -151,7 +151,7 @@ def main():
     if bad:
         raise SystemExit(f"{bad} particles went non-finite")
 
-    # per-kernel times from hipEvents on the library's stream, same number of steps
+    # per-kernel times from dispatch-stamped hipEvents on the library's stream, same number of steps
     pm.set_timing(True)
     for _ in range(args.steps):
         a, dm = pm.step(a, args.da_frac * a)

@@ -119,9 +119,10 @@ int pm3d_poisson_gradient(void* handle, const double* delta, double s, double sm
  * minimum-image unwrapped, in units of the lattice spacing.  Requires Np == particles per side. */
 int pm3d_min_jacobian(void* handle, double* jmin);
 
-/* Per-kernel timing with hipEvents recorded on the handle's stream.  When enabled, every launch
- * of the step is bracketed by events; pm3d_get_timings returns (count, total ms) per kernel class
- * in the order of pm3d_kernel_names(), and resets the accumulators. */
+/* Per-kernel timing with hipEvents on the handle's stream.  When enabled, each step kernel's
+ * dispatch stamps its own start/stop event pair (hipExtLaunchKernelGGL), other launches are
+ * bracketed by event records; pm3d_get_timings returns (count, total ms) per kernel class in the
+ * order of pm3d_kernel_names(), and resets the accumulators. */
 int pm3d_set_timing(void* handle, int enable);
 int pm3d_get_timings(void* handle, int64_t* counts, double* total_ms, int nmax);
 const char* pm3d_kernel_names(void);  /* comma-separated */
