@@ -373,3 +373,44 @@ def test_deferred_gather_is_invisible(PM3D, prec):
         del pm
     for o in outs[1:]:
         assert np.array_equal(outs[0][0], o[0]) and np.array_equal(outs[0][1], o[1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_kernel_timing_is_invisible_and_consistent(PM3D, prec):
+    """Per-kernel timing (each step kernel's dispatch stamping its own start/stop events) leaves the
+    trajectory bit for bit unchanged, times every step kernel once per step, and its durations are
+    positive and add up to no more than the wall time of the steps they were taken in."""
+    import time
+    from pbh_amd import ic
+    say("kernel timing", prec)
+    spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    outs = []
+    for timed in (False, True):
+        pm = PM3D(64, 1.0, 64, precision=prec)
+        pm.set_state(st["x"], st["p"])
+        a = st["a_i"]
+        a, _ = pm.step(a, 0.03 * a)  # first step: forces of the initial state, then the step
+        pm.synchronize()
+        pm.set_timing(timed)
+        pm.timings()  # reset the accumulators
+        steps = 5
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a, _ = pm.step(a, 0.03 * a)
+        pm.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        tm = pm.timings()
+        pm.set_timing(False)
+        outs.append((pm.x, pm.p))
+        if timed:
+            step_kernels = ("gather_kick", "deposit", "deposit_far", "fwd_zy", "x_poisson", "inv_yz")
+            for k in step_kernels:
+                cnt, ms = tm[k]
+                assert cnt == steps, (k, cnt)
+                assert np.isfinite(ms) and 0.0 < ms < wall_ms, (k, ms, wall_ms)
+            assert sum(tm[k][1] for k in step_kernels) <= wall_ms
+        else:
+            assert all(c == 0 for c, _ in tm.values())
+        del pm
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
