@@ -62,7 +62,7 @@ def algorithmic_bytes(N: int, n: int, prec: str) -> dict:
 
 
 def load_traffic(kernel: str, workload: str):
-    f = ROOT / "profiles" / "traffic.json"
+    f = ROOT / "profiles" / "step_kernels_traffic_peak128_fp64_peak256_mixed.json"
     if not f.exists():
         return None
     try:
