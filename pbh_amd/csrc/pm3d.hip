@@ -253,7 +253,8 @@ struct Fft {
     }
     // split 2 ways along y: measured on MI355X against 4 and 8 (smaller tiles, more workgroups per
     // CU, the plane read 4 or 8 times from L2), 128^3 FP64 / 256^3 FP32 inverse pass: 30.2 / 159 us
-    // (2), 32.7 / 189 us (4), 40.1 / 216 us (8, before the bank-conflict fix of the y phase)
+    // (2), 32.7 / 189 us (4), 40.1 / 216 us (8, before the bank-conflict fix of the y phase); with the
+    // twiddles in LDS: 26.8 / 137.5 us (2), 27.7 / 156.4 us (4)
     static constexpr int kInvQ = 2;
     static void inv_plane(Handle* h, cplx<R>* base, int ncomp, void* out) { inv_plane_q<kInvQ>(h, base, ncomp, out); }
 
