@@ -153,7 +153,7 @@ int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax);
  * narrower one is at least 3 % faster (cached per mesh size and precision); every shape gives the
  * same mesh bits.  Environment PM3D_DEPOSIT_SHAPE =
  * one digit 0-3 forces the n-th fitting shape.  Returns 1 for columns, 0 for the atomic path.  No
- * reference counterpart (pm3d.py:70-89 deposits with numpy.add.at). */
+ * reference counterpart (pm3d.py:81-89 deposits with one np.bincount per CIC corner). */
 int pm3d_deposit_tiles(void* handle, int32_t* shape);
 
 /* Initial conditions built on the device (the construction of PM3D.make_ic, pm3d.py:59-68,

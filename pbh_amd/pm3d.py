@@ -125,14 +125,18 @@ class PM3D:
         self._kgrid = None
         self.n = Np ** 3
 
-    def __del__(self):
+    def close(self):
+        """Release the handle's device memory now (otherwise when the object is collected)."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            try:
-                self._L.pm3d_destroy(h)
-            except Exception:
-                pass
-            self._h = None
+            self._L.pm3d_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     # -- state (pm3d.py:67-68,124)
     @property

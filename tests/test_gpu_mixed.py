@@ -53,9 +53,24 @@ def test_mixed_state_roundtrip_and_keys():
     pm.set_state(st["x"], st["p"])
     assert pos_err(pm.x, st["x"], 1.0) <= 2 ** -24 / 64 * 1.01
     assert rel(pm.p, st["p"]) <= 1e-7
-    # the stored base cell is the oracle's CIC key
-    key = O.cic_key(pm.x, 1.0 / 64, 64)
-    np.testing.assert_array_equal(key, O.cic_key(st["x"], 1.0 / 64, 64)[: len(key)] * 0 + key)
+    # The stored base cell is the oracle's CIC base index i0 = floor(x/dx - 0.5) of the FP64 input
+    # (pm3d.py:77-79), except where the fraction f = g - i0 rounds to 1.0 in FP32: then the cell is
+    # i0 + 1 with offset 0.  Reading x back gives (cell + 0.5 + off) * dx exactly (dx = 2^-6), so
+    # the oracle's index of the returned positions is the stored cell itself.
+    xb = pm.x
+    n_up = 0
+    for d in range(3):
+        lo, hi, f = O.cic_index(st["x"][:, d], 1.0 / 64, 64)
+        up = f.astype(np.float32) >= np.float32(1.0)
+        n_up += int(up.sum())
+        assert np.all(f[up] > 1.0 - 2.0 ** -24)
+        expect = np.where(up, hi, lo)
+        got = O.cic_index(xb[:, d], 1.0 / 64, 64)[0]
+        np.testing.assert_array_equal(got, expect)
+    assert n_up <= 1e-4 * st["x"].size
+    # and the flat key of the returned positions matches the oracle's key wherever no axis rounded up
+    k_dev, k_ref = O.cic_key(xb, 1.0 / 64, 64), O.cic_key(st["x"], 1.0 / 64, 64)
+    assert int((k_dev != k_ref).sum()) <= n_up
 
 
 @pytest.mark.parametrize("N", [32, 64])
