@@ -83,6 +83,18 @@ int pm3d_cic_index(void* handle, const double* X, int64_t n, int64_t* key, doubl
 /* PM3D.deposit(X) (pm3d.py:81-89): particle counts per cell, (N,N,N) FP64, from X (n,3). */
 int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho);
 
+/* Allocation-free variant of pm3d_deposit, with no handle: X (n,3) FP64 and rho (N,N,N) FP64 are
+ * DEVICE pointers the caller owns; the call is asynchronous on `stream` (a hipStream_t, NULL = the
+ * null stream).  No counterpart in the reference: it is for callers that cannot allocate (code under
+ * hipGraph capture, tight serving loops, hosts that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or allocating library routine: the fixed-point mesh lives in a device array of the
+ * library of 2 x 128^3 uint64 (32 MiB), held by every GPU the library runs on until the process
+ * ends, so N is limited to 128 (power of two in [8,128], or 48, 96; otherwise PM3D_EINVAL).  The
+ * result is bit for bit that of pm3d_deposit on an FP64 handle (same 2^-52 fixed point).  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialised by the caller.  Errors are read with pm3d_last_error(NULL). */
+int pm3d_deposit_noalloc(const double* X, int64_t n, int32_t N, double L, double* rho, void* stream);
+
 /* PM3D.interp(F, X) (pm3d.py:91-99): CIC gather of the (N,N,N) mesh F at X (n,3) into out (n,). */
 int pm3d_interp(void* handle, const double* F, const double* X, int64_t n, double* out);
 

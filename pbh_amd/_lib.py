@@ -56,6 +56,7 @@ SIGNATURES = {
     "pm3d_get_state": (c_int, [c_void_p, PD, PD]),
     "pm3d_cic_index": (c_int, [c_void_p, PD, c_int64, PI64, PD]),
     "pm3d_deposit": (c_int, [c_void_p, PD, c_int64, PD]),
+    "pm3d_deposit_noalloc": (c_int, [c_void_p, c_int64, c_int32, c_double, c_void_p, c_void_p]),
     "pm3d_interp": (c_int, [c_void_p, PD, PD, c_int64, PD]),
     "pm3d_forces": (c_int, [c_void_p, c_double, c_double, c_double, PD, PD]),
     "pm3d_step": (c_int, [c_void_p, POINTER(Pm3dStepScalars), PD, PD]),

@@ -1107,6 +1107,40 @@ void launch_analyse(Handle* h, const Pos& pos, const AnRadii& radii) {
     k_an_radii<<<grid_cap(h->n), kThreads, 0, h->st>>>(pos, h->n, centre, L, hL, radii, h->an_cnt);
 }
 
+// ------------------------------------------------------- allocation-free deposit (scratch array)
+// pm3d_deposit_noalloc keeps its 128-bit fixed-point mesh in this file-scope device array instead
+// of memory the library allocates: 2 x 128^3 words (32 MiB), held by every GPU the library runs on
+// until the process ends.  In host code `scratch` is only a symbol: its device address on each GPU
+// comes from hipGetSymbolAddress with that GPU current (scratch_on_device), and kernels and fills
+// receive that address, never the array's name.
+constexpr long long kScratchCells = 128LL * 128 * 128;  // largest mesh: N = 128
+__device__ __align__(256) unsigned long long scratch[2 * kScratchCells];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned long long*> g_scratch_addr;  // device ordinal -> address of `scratch` there
+
+int scratch_on_device(unsigned long long** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return PM3D_EHIP;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(dev);
+    if (it == g_scratch_addr.end()) {
+        unsigned long long* ptr = nullptr;
+        e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return PM3D_EHIP;
+        }
+        it = g_scratch_addr.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return 0;
+}
+
 }  // namespace
 
 // ================================================================================ C-ABI
@@ -1357,6 +1391,39 @@ int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho) {
     HCHK(h, hipStreamSynchronize(h->st));
     h->serial += 1;  // the state's last deposit is no longer in the mesh
     return check_launch(h, "deposit");
+}
+
+int pm3d_deposit_noalloc(const double* X, int64_t n, int32_t N, double L, double* rho, void* stream) {
+    if (n < 0 || (n > 0 && !X) || !rho || !(L > 0.0) || !std::isfinite(L)) {
+        g_create_error = "pm3d_deposit_noalloc: invalid argument";
+        return PM3D_EINVAL;
+    }
+    int lg = 0;
+    while ((1 << lg) < N && lg < 30) ++lg;
+    if (!((N >= 8 && (1 << lg) == N) || N == 48 || N == 96) || (long long)N * N * N > kScratchCells) {
+        g_create_error = "pm3d_deposit_noalloc: N must be a power of two in [8,128], or 48 or 96 (scratch holds 128^3 cells)";
+        return PM3D_EINVAL;
+    }
+    if (n > (1LL << 38)) {
+        g_create_error = "pm3d_deposit_noalloc: more than 2^38 particles";
+        return PM3D_EINVAL;
+    }
+    unsigned long long* mesh = nullptr;
+    if (int rc = scratch_on_device(&mesh)) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const long long M = (long long)N * N * N;
+    // the same kernels and constants as pm3d_deposit on an FP64 handle (2^-52 on 128-bit cells)
+    k_fill_u64<<<grid_cap(2 * M), kThreads, 0, st>>>(mesh, 2 * M, 0ull);
+    if (n > 0)
+        k_deposit_atomic<true><<<grid_for(n), kThreads, 0, st>>>(Strided{X, 1, 3}, n, N, L / N, mesh,
+                                                                  std::ldexp(1.0, kWideFB), 1LL << kWideFB);
+    k_fixed_to_real<true><<<grid_cap(M), kThreads, 0, st>>>(mesh, M, std::ldexp(1.0, -kWideFB), 0, 1.0, rho);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_create_error = std::string("pm3d_deposit_noalloc: ") + hipGetErrorString(e);
+        return PM3D_EHIP;
+    }
+    return 0;
 }
 
 int pm3d_interp(void* handle, const double* F, const double* X, int64_t n, double* out) {
@@ -1618,13 +1685,13 @@ int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, 
     if (rho_max) {
         // maximum of a fresh deposit of the state, into the gradient spectra (free between steps), so
         // the density mesh of the last step stays readable
-        unsigned long long* scratch = (unsigned long long*)h->gspec;
-        if ((rc = deposit_into(h, scratch))) return rc;
+        unsigned long long* fresh = (unsigned long long*)h->gspec;
+        if ((rc = deposit_into(h, fresh))) return rc;
         const unsigned nb = (unsigned)std::min<size_t>(1024, h->bmax_len);
         if (h->wide)
-            k_an_mesh_max<true><<<nb, kThreads, 0, h->st>>>(scratch, h->M, h->qscale, h->bmax);
+            k_an_mesh_max<true><<<nb, kThreads, 0, h->st>>>(fresh, h->M, h->qscale, h->bmax);
         else
-            k_an_mesh_max<false><<<nb, kThreads, 0, h->st>>>(scratch, h->M, h->qscale, h->bmax);
+            k_an_mesh_max<false><<<nb, kThreads, 0, h->st>>>(fresh, h->M, h->qscale, h->bmax);
         k_an_max_reduce<<<1, kThreads, 0, h->st>>>(h->bmax, (int)nb, h->an_buf + 3);
         if ((rc = check_launch(h, "analyse rho max"))) return rc;
     }
