@@ -151,6 +151,16 @@ int pm3d_set_analysis_sets(void* handle, const int32_t* idx, const int64_t* offs
 int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, double* moments, int64_t* counts,
                  double* rho_max);
 
+/* Global sums kept in FP64 (or exact) in every precision mode.  mass = total of the most recent
+ * deposit in particle units, summed exactly as a 128-bit integer of fixed-point weights and then
+ * rounded to FP64; mass_dev = that exact total minus n (each corner weight is rounded to the
+ * fixed-point grid on its own, so the deposit's mass is n to within that rounding); momentum[3] =
+ * sum over particles of p (after the pending closing kick), accumulated in FP64 over a fixed
+ * reduction tree (the same bits for the same state).  Any output may be NULL.  The reference
+ * conserves mass and momentum implicitly (np.bincount of unit weights, pm3d.py:88; the momentum
+ * update pm3d.py:122,127) and has no such diagnostic. */
+int pm3d_totals(void* handle, double* mass, double* mass_dev, double* momentum);
+
 /* FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z), two bits: bit 0 = pencils per
  * workgroup sized for 32 KB rather than 16 KB LDS buffers, bit 1 = one buffer transformed in place
  * rather than two ping-pong buffers.  Every variant gives the same bits; pm3d_create times all four

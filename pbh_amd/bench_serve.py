@@ -15,11 +15,20 @@ from . import ic
 from .serve import BisectionChain, CollapseServer, Request, config_requests, poisson_arrivals, summarize
 
 
+def _valid(spec) -> bool:
+    try:
+        ic.validate(spec)
+        return True
+    except ValueError:
+        return False
+
+
 def plain_fp64(specs):
     """One request at a time, FP64, PM3D.step + min det(dx/dq) per step (initial conditions on the device)."""
     t0 = time.perf_counter()
     out = []
     eng = {}
+    specs = [s for s in specs if _valid(s)]
     for s in specs:
         pm = ic.device_ic(PM3D(s.N, 1.0, s.N, precision="fp64"), s)
         out.append(run_to_crossing(pm, s.a_i, 0.02, 1.0))

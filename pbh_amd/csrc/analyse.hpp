@@ -205,4 +205,54 @@ __global__ __launch_bounds__(kThreads) void k_an_max_reduce(const double* __rest
     if (threadIdx.x == 0) *out = m;
 }
 
+// ------------------------------------------------------------------------------- global sums
+// The global sums that stay in FP64 (or exact) in every precision mode.  Mass: the total of the
+// fixed-point mesh of a deposit, summed as a 128-bit integer (exact in any order); each block
+// stores its partial total as {lo, hi}.  Momentum: sum of p_i per axis in FP64 over a fixed grid
+// and a fixed tree, so the same state always gives the same bits.
+constexpr int kTotBlocks = 1024;
+
+template <bool WIDE>
+__global__ __launch_bounds__(kThreads) void k_mesh_total(const unsigned long long* __restrict__ mesh, long long M,
+                                                         unsigned long long* __restrict__ part) {
+    unsigned __int128 s = 0;
+    for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < M; c += (long long)gridDim.x * kThreads) {
+        if constexpr (WIDE)
+            s += ((unsigned __int128)mesh[2 * c + 1] << 64) | (unsigned __int128)mesh[2 * c];
+        else
+            s += (unsigned __int128)mesh[c];
+    }
+    __shared__ unsigned long long lo[kThreads], hi[kThreads];
+    lo[threadIdx.x] = (unsigned long long)s;
+    hi[threadIdx.x] = (unsigned long long)(s >> 64);
+    __syncthreads();
+    for (int st = kThreads / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const unsigned __int128 a = ((unsigned __int128)hi[threadIdx.x] << 64) | lo[threadIdx.x];
+            const unsigned __int128 b = ((unsigned __int128)hi[threadIdx.x + st] << 64) | lo[threadIdx.x + st];
+            const unsigned __int128 t = a + b;
+            lo[threadIdx.x] = (unsigned long long)t;
+            hi[threadIdx.x] = (unsigned long long)(t >> 64);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = lo[0];
+        part[2 * blockIdx.x + 1] = hi[0];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_momentum_total(const T* __restrict__ p, long long n,
+                                                             double* __restrict__ part) {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] += (double)p[d * n + i];
+    }
+    __shared__ double red[3];
+    an_block_sum<3>(v, red);
+    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x];
+}
+
 }  // namespace pm3d

@@ -1708,6 +1708,52 @@ int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, 
     return 0;
 }
 
+int pm3d_totals(void* handle, double* mass, double* mass_dev, double* momentum) {
+    if (!handle) return PM3D_EINVAL;
+    Handle* h = (Handle*)handle;
+    if (!h->state_set) return fail(h, PM3D_ESTATE, "state not set");
+    if (h->serial < 1 && (mass || mass_dev)) return fail(h, PM3D_ESTATE, "no deposit yet");
+    int rc = flush_kick(h);  // the momenta after the pending closing kick (same bits later on)
+    if (rc) return rc;
+    rc = ensure_sbuf(h, (size_t)kTotBlocks * 5 * sizeof(double));
+    if (rc) return rc;
+    unsigned long long* mpart = (unsigned long long*)h->sbuf;
+    double* ppart = h->sbuf + 2 * kTotBlocks;
+    if (h->wide)
+        k_mesh_total<true><<<kTotBlocks, kThreads, 0, h->st>>>(h->mesh, h->M, mpart);
+    else
+        k_mesh_total<false><<<kTotBlocks, kThreads, 0, h->st>>>(h->mesh, h->M, mpart);
+    if (h->mixed)
+        k_momentum_total<float><<<kTotBlocks, kThreads, 0, h->st>>>(h->pf, h->n, ppart);
+    else
+        k_momentum_total<double><<<kTotBlocks, kThreads, 0, h->st>>>(h->p, h->n, ppart);
+    if ((rc = check_launch(h, "totals"))) return rc;
+    std::vector<unsigned long long> hm(2 * kTotBlocks);
+    std::vector<double> hp(3 * kTotBlocks);
+    HCHK(h, hipMemcpyAsync(hm.data(), mpart, hm.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipMemcpyAsync(hp.data(), ppart, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipStreamSynchronize(h->st));
+    unsigned __int128 tot = 0;
+    for (int b = 0; b < kTotBlocks; ++b) tot += ((unsigned __int128)hm[2 * b + 1] << 64) | hm[2 * b];
+    const int fb = h->cfg.frac_bits;
+    // n particles of unit weight would total exactly n * 2^fb
+    const unsigned __int128 ideal = (unsigned __int128)h->n << fb;
+    const bool over = tot >= ideal;
+    const unsigned __int128 dev = over ? tot - ideal : ideal - tot;
+    auto to_double = [fb](unsigned __int128 v) {
+        return std::ldexp((double)(unsigned long long)(v >> 64), 64 - fb) + std::ldexp((double)(unsigned long long)v, -fb);
+    };
+    if (mass) *mass = to_double(tot);
+    if (mass_dev) *mass_dev = over ? to_double(dev) : -to_double(dev);
+    if (momentum) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int b = 0; b < kTotBlocks; ++b)
+            for (int d = 0; d < 3; ++d) s[d] += hp[3 * b + d];
+        for (int d = 0; d < 3; ++d) momentum[d] = s[d];
+    }
+    return 0;
+}
+
 int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax) {
     if (!handle || !variants || nmax < 0) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;

@@ -192,6 +192,23 @@ def batch_spec(seed: int) -> PeakSpec:
     return PeakSpec(N=N, mu=mu, sigma=tuple(ax), seed=seed, name=f"batch{seed}", meta={"e": e, "p": p})
 
 
+def validate(spec: PeakSpec) -> None:
+    """Raise ValueError for parameters that cannot describe a peak: a non-finite amplitude, widths
+    that are not finite and positive (the Zel'dovich axes are undefined when the smallest
+    eigenvalue (1 - 3e + p)/3 is not positive), a non-finite random field, or a bad start."""
+    if not np.isfinite(spec.mu):
+        raise ValueError(f"{spec.name or 'peak'}: amplitude mu={spec.mu} is not finite")
+    sig = np.asarray(spec.sigma, dtype=float)
+    if sig.shape != (3,) or not np.all(np.isfinite(sig)) or not np.all(sig > 0):
+        e, p = spec.meta.get("e"), spec.meta.get("p")
+        raise ValueError(f"{spec.name or 'peak'}: widths {tuple(sig)} are not finite and positive"
+                         + (f" (Zel'dovich axes undefined for e={e:g}, p={p:g})" if e is not None else ""))
+    if not (np.isfinite(spec.noise_rms_frac) and spec.noise_rms_frac >= 0 and np.isfinite(spec.noise_kmax_frac)):
+        raise ValueError(f"{spec.name or 'peak'}: random field parameters are not finite")
+    if not (np.isfinite(spec.a_i) and spec.a_i > 0 and np.isfinite(spec.L) and spec.L > 0):
+        raise ValueError(f"{spec.name or 'peak'}: a_i and L must be finite and positive")
+
+
 def device_ic(pm, spec: PeakSpec, engine=None):
     """Build the workload's initial state on the GPU (pm3d_make_peak_ic) in `pm`.
 
@@ -200,6 +217,7 @@ def device_ic(pm, spec: PeakSpec, engine=None):
     import ctypes
     from . import _lib
     from .pm3d import PM3D
+    validate(spec)
     if spec.n_side != spec.N:
         raise ValueError("device initial conditions need Np == N")
     ic = _lib.Pm3dPeakIC()

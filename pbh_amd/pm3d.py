@@ -355,6 +355,14 @@ class PM3D:
         check(self._L.pm3d_min_jacobian(self._h, ctypes.byref(v)), self._h)
         return float(v.value)
 
+    def totals(self) -> dict:
+        """Global sums (FP64 / exact in every precision mode): mass of the most recent deposit in
+        particle units and its exact deviation from n, and the total momentum sum_i p_i."""
+        m, dev = ctypes.c_double(), ctypes.c_double()
+        mom = np.zeros(3)
+        check(self._L.pm3d_totals(self._h, ctypes.byref(m), ctypes.byref(dev), dptr(mom)), self._h)
+        return {"mass": m.value, "mass_dev": dev.value, "momentum": mom}
+
     def device_bytes(self) -> int:
         return int(self._L.pm3d_device_bytes(self._h))
 

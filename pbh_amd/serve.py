@@ -190,6 +190,12 @@ class CollapseServer:
     # -- request lifecycle
     def _admit(self, req: Request, now: float) -> bool:
         spec = req.spec
+        try:
+            ic.validate(spec)  # an invalid request fails here, before any slot or device work
+        except ValueError as err:
+            req.t_admit = now
+            self._fail(req, None, f"ValueError: {err}", now)
+            return True
         eng = self._engine(spec.N)
         if eng is False:
             return False
@@ -226,7 +232,8 @@ class CollapseServer:
         req.t_done = now
         self.failed.append(req)
         self.completed.append(req)
-        self._release(pm)  # its state is not trusted: never reused
+        if pm is not None:
+            self._release(pm)  # its state is not trusted: never reused
 
     def _complete(self, req: Request, m: Member, now: float):
         bad = m.pm.synchronize()
@@ -335,7 +342,8 @@ def config_requests(n: int, N: int = 256, seed0: int = 1000, noise: float = 0.01
         mu = float(rng.uniform(0.05, 0.5))
         e = float(rng.uniform(0.0, 0.3))
         p = float(rng.uniform(-e, e))
-        ax = 0.1 * ic.zeldovich_axes(e, p)
+        with np.errstate(invalid="ignore"):  # 1 - 3e + p <= 0: undefined axes, rejected at admission
+            ax = 0.1 * ic.zeldovich_axes(e, p)
         out.append(ic.PeakSpec(N=N, mu=mu, sigma=tuple(ax), seed=seed0 + k, noise_rms_frac=noise, name=f"req{k}",
                                meta={"e": e, "p": p}))
     return out

@@ -414,3 +414,43 @@ def test_kernel_timing_is_invisible_and_consistent(PM3D, prec):
             assert all(c == 0 for c, _ in tm.values())
         del pm
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_global_sums(PM3D, prec):
+    """pm3d_totals: the deposit's mass is n plus the exact deviation of the fixed-point total
+    (FP64 mode: recomputed on the host as the integer sum over particles of sum_c rint(w_c 2^52) -
+    2^52 from the oracle's corner weights, which equal the kernel's on identical positions; the
+    weights of a particle need not sum to exactly 1 in FP64), and the total momentum matches the
+    oracle's sum of p; the same state gives the same bits twice."""
+    from pbh_amd import ic
+    say("global sums", prec)
+    spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    pm = PM3D(64, 1.0, 64, precision=prec)
+    pm.set_state(st["x"], st["p"])
+    ref = O.OraclePM3D(64, 1.0, 64)
+    ref.x, ref.p = st["x"].copy(), st["p"].copy()
+    a = st["a_i"]
+    for _ in range(5):
+        pm.step(a, 0.03 * a)
+        a, _ = ref.step(a, 0.03 * a)
+    t1, t2 = pm.totals(), pm.totals()
+    assert t1["mass"] == t2["mass"] and t1["mass_dev"] == t2["mass_dev"]
+    assert np.array_equal(t1["momentum"], t2["momentum"])
+    n = 64 ** 3
+    assert t1["mass"] == pytest.approx(n + t1["mass_dev"], rel=1e-15)
+    scale = np.abs(ref.p).sum(axis=0)
+    dmom = np.abs(t1["momentum"] - ref.p.sum(axis=0)) / scale
+    say(f"{prec}: mass - n = {t1['mass_dev']:.3e}, momentum {t1['momentum']}, rel diff {dmom.max():.2e}")
+    if prec == "fp64":
+        x = pm.x
+        units = np.zeros(len(x), np.int64)  # per particle sum_c q_c <= 8 * 2^52: exact in int64
+        for ix, iy, iz, wx, wy, wz in O._corners(x, 1.0 / 64, 64):
+            units += np.rint((wx * wy * wz) * 2.0 ** 52).astype(np.int64)
+        dev_units = int((units - (1 << 52)).sum())
+        assert t1["mass_dev"] == dev_units * 2.0 ** -52
+        assert dmom.max() <= 1e-12
+    else:
+        assert abs(t1["mass_dev"]) <= 8 * n * 2.0 ** -32
+        assert dmom.max() <= 1e-5

@@ -187,3 +187,19 @@ def test_chain_probes_follow_completions():
     assert len(done) == 5 and ch.done == 5
     assert len({h[0] for h in ch.history}) == 5  # every probe bisects to a new mu
     assert np.all(np.diff([r.t_done for r in done]) >= 0)
+
+
+@pytest.mark.timeout(30)
+def test_invalid_request_fails_before_any_slot():
+    """A request whose Zel'dovich axes are undefined (1 - 3e + p <= 0 gives NaN widths) fails at
+    admission without taking a slot or building anything; the valid one completes."""
+    StubPM.live = 0
+    srv = make_server(budget=10_000, sizes={32: 100})
+    with np.errstate(invalid="ignore"):
+        ax = tuple(0.1 * ic.zeldovich_axes(0.3, -0.3))
+    bad = ic.PeakSpec(N=32, mu=0.2, sigma=ax, meta={"e": 0.3, "p": -0.3})
+    done = {r.rid: r for r in srv.run([(0.0, Request(rid=1, spec=bad, arrival=0.0)),
+                                       (0.0, Request(rid=2, spec=spec(32, 0.2), arrival=0.0))])}
+    assert "Zel'dovich axes undefined" in done[1].error and done[1].result is None
+    assert done[2].error is None and done[2].result.crossed
+    assert StubPM.live == 2  # one slot and its engine: the invalid request allocated nothing
