@@ -37,6 +37,7 @@ struct MxBox {
     const cplx<float>* tw;
     double* bmax;                // per-block maxima / minima
     unsigned long long* qslot;   // deposit maximum slot
+    FarRows fr;                  // far-particle row lists (cnt == nullptr: none)
     double kval;                 // 1/(N dx)
     double inv_c3;               // (Np / N)^3: 1 / (cells per lattice site)^3
     double qscale, nmean;
@@ -54,14 +55,15 @@ __device__ __forceinline__ SrcMx box_src(const MxBox& b) {
 __global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
     if (box_done(b)) return;
-    gather_kick_drift_mx_body(b.cell, b.off, b.p, b.n, b.N, b.grad, 1, 0.f, 0.f, 0.f, b.dscal + 1, b.sched, b.tab, xm);
+    gather_kick_drift_mx_body(b.cell, b.off, b.p, b.n, b.N, b.grad, 1, 0.f, 0.f, 0.f, b.dscal + 1, b.sched, b.tab, xm,
+                              b.fr);
 }
 
 template <int BX, int BY, int N>
 __global__ __launch_bounds__(1024) void k_deposit_col_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox& b = boxes[blockIdx.y];
     if (box_done(b)) return;
-    deposit_col_body<SrcMx, BX, BY, N, false, 1024>(box_src(b), b.mesh, b.far, b.farcnt, xm);
+    deposit_col_body<SrcMx, BX, BY, N, false, 1024>(box_src(b), b.mesh, b.far, b.farcnt, b.fr, xm);
 }
 
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox* __restrict__ boxes, int seg) {
@@ -73,7 +75,8 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox*
 __global__ __launch_bounds__(kThreads) void k_deposit_far_bal_boxes(const MxBox* __restrict__ boxes, int ncol, int seg) {
     const MxBox& b = boxes[blockIdx.y];
     if (box_done(b)) return;
-    deposit_far_bal_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, ncol, seg, b.N, blockIdx.x, gridDim.x);
+    deposit_far_bal_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, ncol, seg, b.N, blockIdx.x, gridDim.x, b.fr,
+                                       1);
 }
 
 template <int N>
@@ -81,7 +84,8 @@ __global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(c
     const MxBox& b = boxes[blockIdx.y];
     if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2);
-    fwd_plane_body<float, N, true, false>(b.mesh, b.spec, b.spec + specN, b.tw, b.bmax, b.qscale, b.nmean, xm);
+    fwd_plane_body<float, N, true, false>(b.mesh, b.spec, b.spec + specN, b.tw, b.bmax, b.qscale, b.nmean, xm,
+                                          b.fr.ovcnt);
 }
 
 // The Poisson x pass with the scheduled Green's function factor; its last block reduces the

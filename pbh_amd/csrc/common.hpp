@@ -198,6 +198,83 @@ __device__ __forceinline__ double block_min(double v) {
     return from_ordered_bits(m);
 }
 
+// ------------------------------------------------------------------ far-particle row lists
+// The column deposit takes a particle from its Lagrangian site when its CIC base cell is the site
+// or the site below it on every axis ("near").  The others ("far": 19-42 % of the particles at the
+// first shell crossing of a collapsing peak, most 2-10 cells from their sites) are sorted by the
+// high part of their cell key, the row r = ix0*N + iy0, by the kernel that moves them (the
+// gather-kick-drift of the same step): each is appended to the list of its row, and the column
+// deposit reads the lists of the rows its column and low ring cover, so far particles are
+// accumulated in LDS like near ones.  A row holds `cap` entries; the rest go to a flat overflow
+// list deposited with global atomics.  The row counts are cleared by the far-particle pass of the
+// deposit that consumes them, the overflow count by the forward transform that follows it.
+struct FarRows {
+    unsigned int* cnt;    // [N*N] entries appended per row (may exceed cap); nullptr: no lists
+    int* list;            // [N*N][cap] particle indices
+    int cap;
+    int* ovl;             // overflow entries, flat
+    unsigned int* ovcnt;  // their count (appends past ovcap are counted but not stored)
+    unsigned int ovcap;   // entries ovl holds
+};
+
+// Appends particle i to its row's list for the lanes with `far` set.  The lanes of a wave are
+// consecutive lattice sites, whose far particles come in runs of lanes with one row: each run takes
+// its slots with one atomic (all runs of the wave in one instruction), and a lane's slot is its
+// run's base plus its place in the run.  Must be called by every lane of the wave.
+__device__ __forceinline__ void far_rows_append(const FarRows& fr, bool far, unsigned row, int i) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long fm = __ballot(far);
+    if (fm == 0ull) return;
+    const unsigned prow = (unsigned)__shfl_up((int)row, 1u, 64);
+    const bool head = far && (lane == 0 || !((fm >> ((lane + 63) & 63)) & 1ull) || prow != row);
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long below = (2ull << lane) - 1ull;  // lanes 0..lane (all 64 for lane 63)
+    unsigned base = 0;
+    if (head) {  // the run ends at the next head or the next lane that is not far
+        const unsigned long long after = (hm | ~fm) & ~below;
+        const int end = after ? __ffsll((long long)after) - 1 : 64;
+        base = atomicAdd(fr.cnt + row, (unsigned)(end - lane));
+    }
+    const int hl = far ? 63 - __clzll((long long)(hm & below)) : lane;  // this lane's run head
+    const unsigned hbase = (unsigned)__shfl((int)base, hl, 64);
+    bool ov = false;
+    if (far) {
+        const unsigned slot = hbase + (unsigned)(lane - hl);
+        if (slot < (unsigned)fr.cap) fr.list[(size_t)row * fr.cap + slot] = i;
+        else ov = true;
+    }
+    const unsigned long long ovm = __ballot(ov);
+    if (ovm) {
+        const int leader = __ffsll((long long)ovm) - 1;
+        unsigned obase = 0;
+        if (lane == leader) obase = atomicAdd(fr.ovcnt, (unsigned)__popcll(ovm));
+        obase = (unsigned)__shfl((int)obase, leader, 64);
+        const unsigned slot = obase + (unsigned)__popcll(ovm & ((1ull << lane) - 1ull));
+        if (ov && slot < fr.ovcap) fr.ovl[slot] = i;
+    }
+}
+
+// The column deposit's "near" test on one axis for an FP64 coordinate (the shared definition of
+// the deposit and of the kernel that sorts the far particles): g = X/dx - 0.5 with the
+// Markstein-corrected quotient of div_dx, lo = floor(g) mod N, f = g - floor(g); near when the base
+// cell is site s or the cell below it (periodic).  |X| >= 1e290 or NaN gives not near.
+__device__ __forceinline__ bool near_f64(double X, double dx, double inv_dx, int N, int s, int& lo, double& f) {
+    const double q0 = X * inv_dx;
+    const double q = fma(fma(-q0, dx, X), inv_dx, q0);
+    const double g = q - 0.5;
+    const double fl = floor(g);
+    f = g - fl;
+    const bool inr = fl >= -1.0 && fl <= (double)(N - 1);
+    const int l = (int)(inr ? fl : 0.0);
+    lo = l < 0 ? l + N : l;
+    const int o = lo - s;
+    return inr && (o == 0 || o == -1 || o == N - 1);
+}
+__device__ __forceinline__ bool near_cell(int lo, int N, int s) {
+    const int o = lo - s;
+    return (unsigned)lo < (unsigned)N && (o == 0 || o == -1 || o == N - 1);
+}
+
 // ----------------------------------------------------------------- deterministic fixed point
 // Cell accumulators of the deposit.  Integer addition commutes, so every launch order gives the
 // same bits.

@@ -53,18 +53,13 @@ struct SrcF64 {
     // huge or NaN and the particle is sent to the far pass, and for |X| <= 1e-290 any quotient
     // within a few ulps gives g == -0.5 exactly.  `ob` is the base cell's offset from site s (0 or -1).
     __device__ __forceinline__ bool near_axis(const Raw& r, int d, int s, int& ob, int& lo, double& f) const {
-        const double X = r.v[d];
-        const double q0 = X * inv_dx;
-        const double q = fma(fma(-q0, dx, X), inv_dx, q0);
-        const double g = q - 0.5;
-        const double fl = floor(g);
-        f = g - fl;
-        const bool inr = fl >= -1.0 && fl <= (double)(N - 1);
-        const int l = (int)(inr ? fl : 0.0);
-        lo = l < 0 ? l + N : l;
-        const int o = lo - s;
-        ob = o == 0 ? 0 : -1;
-        return inr && (o == 0 || o == -1 || o == N - 1);
+        const bool nr = near_f64(r.v[d], dx, inv_dx, N, s, lo, f);
+        ob = lo == s ? 0 : -1;
+        return nr;
+    }
+    // base cell and fraction of one axis (a particle taken from a row list: its row is known)
+    __device__ __forceinline__ void cell_frac(const Raw& r, int d, int& lo, double& f) const {
+        near_f64(r.v[d], dx, inv_dx, N, 0, lo, f);
     }
     __device__ __forceinline__ void axes(const Raw& r, int lo[3], int hi[3], double w0[3], double w1[3]) const {
 #pragma unroll
@@ -109,11 +104,14 @@ struct SrcMx {
     __device__ __forceinline__ bool near_axis(const Raw& r, int d, int s, int& ob, int& lo, float& f) const {
         lo = r.c[d];
         f = r.f[d];
-        const int o = lo - s;
-        ob = o == 0 ? 0 : -1;
-        const bool inr = (unsigned)lo < (unsigned)N;
-        lo = inr ? lo : 0;
-        return inr && (o == 0 || o == -1 || o == N - 1);
+        const bool nr = near_cell(lo, N, s);
+        ob = lo == s ? 0 : -1;
+        lo = (unsigned)lo < (unsigned)N ? lo : 0;
+        return nr;
+    }
+    __device__ __forceinline__ void cell_frac(const Raw& r, int d, int& lo, float& f) const {
+        lo = (unsigned)r.c[d] < (unsigned)N ? r.c[d] : 0;
+        f = r.f[d];
     }
     __device__ __forceinline__ void axes(const Raw& r, int lo[3], int hi[3], float w0[3], float w1[3]) const {
 #pragma unroll
@@ -132,35 +130,91 @@ struct SrcMx {
     __device__ __forceinline__ unsigned quant_scaled(float ws) const { return (unsigned)rintf(ws); }
 };
 
+// The CIC corners of one particle that land in the column, added to the LDS accumulator: base cell
+// at column-local (lx, ly) (-1 .. BX, -1 .. BY: the ring) and z cell lz (in [0, N) for every lane),
+// fractions f.  A corner plane outside the column gets weight 0 and an in-column address; an (x, y)
+// corner pair whose weight is 0 in every lane of the wave (a ring row's far plane, a zero fraction,
+// an idle lane) is skipped, and within a pair only lanes with a non-zero weight issue the atomic.
+// Each corner weight (wx*wy)*wz is rounded to the fixed-point grid on its own.  Called by every lane
+// of the wave (`valid` false for idle lanes).
+template <class Src, int BX, int BY, int NZ, bool WIDE>
+__device__ __forceinline__ void col_add(const Src& src, unsigned long long* acc, unsigned long long* acch, bool valid,
+                                        int lx, int ly, int lz, const typename Src::W f[3]) {
+    using W = typename Src::W;
+    const bool vx0 = valid && (unsigned)lx < (unsigned)BX, vx1 = valid && (unsigned)(lx + 1) < (unsigned)BX;
+    const bool vy0 = (unsigned)ly < (unsigned)BY, vy1 = (unsigned)(ly + 1) < (unsigned)BY;
+    const W one = W(1);
+    const W wx[2] = {vx0 ? (one - f[0]) * Src::kScale : W(0), vx1 ? f[0] * Src::kScale : W(0)};
+    const W wy[2] = {vy0 ? one - f[1] : W(0), vy1 ? f[1] : W(0)};
+    const W wz[2] = {one - f[2], f[2]};
+    const int px[2] = {vx0 ? lx : 0, vx1 ? lx + 1 : 0};
+    const int py[2] = {vy0 ? ly : 0, vy1 ? ly + 1 : 0};
+#pragma unroll
+    for (int cxy = 0; cxy < 4; ++cxy) {
+        const int cx = cxy >> 1, cy = cxy & 1;
+        const W wxy = wx[cx] * wy[cy];
+        if (__ballot(wxy != W(0)) == 0ull) continue;
+        const int a = (px[cx] * BY + py[cy]) * NZ + lz;
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+            const W w = wxy * wz[cz];
+            // only lanes with a contribution add: idle lanes (far particles at their sites, corners
+            // outside the column) would otherwise pile zero adds onto shared LDS addresses
+            if (w == W(0)) continue;
+            if constexpr (WIDE) {
+                unsigned l32, h32;
+                src.quant2(w, l32, h32);
+                atomicAdd(&acc[a + cz], (unsigned long long)l32);
+                atomicAdd(&acch[a + cz], (unsigned long long)h32);
+            } else {
+                atomicAdd(&acc[a + cz], (unsigned long long)src.quant_scaled(w));
+            }
+        }
+    }
+}
+
 // Column deposit.  LDS holds the column as BX*BY rows of N+1 cells: entry N of a row stands for
 // z = 0 (the upper z corner of a particle whose base cell is N-1) and is folded into entry 0 before
 // the store, so the upper z corner is always the next entry.  Threads walk the (BX+2) x (BY+2) x N
 // sites of the column and its ring; when 64 | N a wave covers 64 sites of one row, so the row, the
-// site coordinates and the ownership test are wave-uniform.  A corner plane outside the column gets
-// weight 0 and an in-column address, so every lane runs the same instructions; an (x, y) corner pair
-// whose weight is 0 in every lane of the wave (a ring row's far plane, a zero fraction) is skipped.
+// site coordinates and the ownership test are wave-uniform.  Far particles: with row lists
+// (fr.cnt != nullptr, built by this step's gather-kick-drift) the column skips them at their sites
+// and takes them from the lists of the (BX+1) x (BY+1) base-cell rows whose corners can reach it
+// (the column and its low ring); without, the owner of a far particle's site records it in its
+// segment of `far` for the global-atomic pass.
 template <class Src, int BX, int BY, int N, bool WIDE, int NT>
 __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __restrict__ mesh,
-                                                 int* __restrict__ far, unsigned int* __restrict__ farcnt, XMap xm) {
+                                                 int* __restrict__ far, unsigned int* __restrict__ farcnt, FarRows fr,
+                                                 XMap xm) {
     using W = typename Src::W;
     constexpr int NZ = N + 1;
     constexpr int NCELL = BX * BY * NZ;
     constexpr int EX = BX + 2, EY = BY + 2, NSITE = EX * EY * N;
     constexpr bool ROWS = N % 64 == 0;
+    constexpr int RY = BY + 1, NR = (BX + 1) * RY;  // base-cell rows whose corners reach the column
+    static_assert(NR < NT, "one thread per row list");
     // WIDE: the 128-bit cell is accumulated as two 64-bit sums of the low and high 32-bit halves of
     // each contribution (no return value, no carry test per add) and recombined exactly at the end
     __shared__ unsigned long long acc[NCELL];
     __shared__ unsigned long long acch[WIDE ? NCELL : 1];
     __shared__ unsigned int nfar;
+    __shared__ unsigned int rpre[NR + 1];
+    const bool lists = fr.cnt != nullptr;
+    const int col = (int)xcd_map(blockIdx.x, gridDim.x, xm);  // column id, also this column's far segment
+    constexpr int NBY = N / BY;
+    const int x0 = (col / NBY) * BX, y0 = (col % NBY) * BY;
+    unsigned int rcount = 0;
+    if (lists && (int)threadIdx.x < NR) {  // issued first: the loads complete under the site loop
+        const int ex = threadIdx.x / RY, ey = threadIdx.x % RY;
+        const unsigned c = fr.cnt[wrapN(x0 + ex - 1, N) * N + wrapN(y0 + ey - 1, N)];
+        rcount = c < (unsigned)fr.cap ? c : (unsigned)fr.cap;
+    }
     for (int t = threadIdx.x; t < NCELL; t += NT) {
         acc[t] = 0ull;
         if (WIDE) acch[t] = 0ull;
     }
     if (threadIdx.x == 0) nfar = 0u;
     __syncthreads();
-    constexpr int NBY = N / BY;
-    const int col = (int)xcd_map(blockIdx.x, gridDim.x, xm);  // column id, also this column's far segment
-    const int x0 = (col / NBY) * BX, y0 = (col % NBY) * BY;
     int* myfar = far + (size_t)col * (BX * BY * N);
     const int lane = threadIdx.x & 63;
     const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
@@ -201,7 +255,7 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
             const bool nz = src.near_axis(raw[u], 2, sz, ob[2], lo[2], f[2]);
             const bool is_near = nx && ny && nz;
             const bool own = ex >= 1 && ex <= BX && ey >= 1 && ey <= BY;
-            const bool app = !is_near && own;
+            const bool app = !is_near && own && !lists;
             const unsigned long long bal = __ballot(app);
             if (bal) {  // one LDS atomic per wave, slots by prefix popcount
                 const int leader = __ffsll((long long)bal) - 1;
@@ -211,33 +265,62 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
                 if (app) myfar[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)((sx * N + sy) * N + sz);
             }
             // column-local planes of the base cell: ex-1+ob in [-2, BX]
-            const int lx = ex - 1 + ob[0], ly = ey - 1 + ob[1];
-            const bool vx0 = is_near && (unsigned)lx < (unsigned)BX, vx1 = is_near && (unsigned)(lx + 1) < (unsigned)BX;
-            const bool vy0 = (unsigned)ly < (unsigned)BY, vy1 = (unsigned)(ly + 1) < (unsigned)BY;
-            const W one = W(1);
-            const W wx[2] = {vx0 ? (one - f[0]) * Src::kScale : W(0), vx1 ? f[0] * Src::kScale : W(0)};
-            const W wy[2] = {vy0 ? one - f[1] : W(0), vy1 ? f[1] : W(0)};
-            const W wz[2] = {one - f[2], f[2]};
-            const int px[2] = {vx0 ? lx : 0, vx1 ? lx + 1 : 0};
-            const int py[2] = {vy0 ? ly : 0, vy1 ? ly + 1 : 0};
+            col_add<Src, BX, BY, NZ, WIDE>(src, acc, acch, is_near, ex - 1 + ob[0], ey - 1 + ob[1], lo[2], f);
+        }
+    }
+    if (lists) {
+        // exclusive prefix of the row counts (loaded above), then the entries of all rows dealt
+        // out over the threads; an entry's row by binary search of the prefix
+        if ((int)threadIdx.x < NR) rpre[threadIdx.x] = rcount;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned run = 0;
+            for (int r = 0; r < NR; ++r) {
+                const unsigned c = rpre[r];
+                rpre[r] = run;
+                run += c;
+            }
+            rpre[NR] = run;
+        }
+        __syncthreads();
+        const unsigned total = rpre[NR];
+        constexpr int UL = 4;  // entries per thread whose loads are in flight together
+        for (unsigned t0 = 0; t0 < total; t0 += NT * UL) {
+            int ra[UL], idx[UL];
+            bool valid[UL];
 #pragma unroll
-            for (int cxy = 0; cxy < 4; ++cxy) {
-                const int cx = cxy >> 1, cy = cxy & 1;
-                const W wxy = wx[cx] * wy[cy];
-                if (__ballot(wxy != W(0)) == 0ull) continue;
-                const int a = (px[cx] * BY + py[cy]) * NZ + lo[2];
-#pragma unroll
-                for (int cz = 0; cz < 2; ++cz) {
-                    const W w = wxy * wz[cz];
-                    if constexpr (WIDE) {
-                        unsigned l32, h32;
-                        src.quant2(w, l32, h32);
-                        atomicAdd(&acc[a + cz], (unsigned long long)l32);
-                        atomicAdd(&acch[a + cz], (unsigned long long)h32);
-                    } else {
-                        atomicAdd(&acc[a + cz], (unsigned long long)src.quant_scaled(w));
+            for (int u = 0; u < UL; ++u) {
+                const unsigned t = t0 + u * NT + threadIdx.x;
+                valid[u] = t < total;
+                int a = 0;
+                if (valid[u]) {
+                    int b = NR - 1;
+                    while (a < b) {
+                        const int m = (a + b + 1) >> 1;
+                        if (rpre[m] <= t) a = m;
+                        else b = m - 1;
                     }
                 }
+                ra[u] = a;
+                const int row = wrapN(x0 + a / RY - 1, N) * N + wrapN(y0 + a % RY - 1, N);
+                idx[u] = valid[u] ? fr.list[(size_t)row * fr.cap + (t - rpre[a])] : 0;
+            }
+            typename Src::Raw raw[UL];
+#pragma unroll
+            for (int u = 0; u < UL; ++u)
+                if (valid[u]) raw[u] = src.load((unsigned)idx[u]);
+#pragma unroll
+            for (int u = 0; u < UL; ++u) {
+                if (t0 + u * NT >= total) break;  // wave-uniform: no lane has an entry
+                int lz = 0;
+                W f[3] = {W(0), W(0), W(0)};
+                if (valid[u]) {
+                    int lxy;
+                    src.cell_frac(raw[u], 0, lxy, f[0]);
+                    src.cell_frac(raw[u], 1, lxy, f[1]);
+                    src.cell_frac(raw[u], 2, lz, f[2]);
+                }
+                col_add<Src, BX, BY, NZ, WIDE>(src, acc, acch, valid[u], ra[u] / RY - 1, ra[u] % RY - 1, lz, f);
             }
         }
     }
@@ -266,8 +349,9 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
 
 template <class Src, int BX, int BY, int N, bool WIDE, int NT>
 __global__ __launch_bounds__(NT) void k_deposit_col(Src src, unsigned long long* __restrict__ mesh,
-                                                    int* __restrict__ far, unsigned int* __restrict__ farcnt, XMap xm) {
-    deposit_col_body<Src, BX, BY, N, WIDE, NT>(src, mesh, far, farcnt, xm);
+                                                    int* __restrict__ far, unsigned int* __restrict__ farcnt,
+                                                    FarRows fr, XMap xm) {
+    deposit_col_body<Src, BX, BY, N, WIDE, NT>(src, mesh, far, farcnt, fr, xm);
 }
 
 // One far particle's eight corners, added with global atomics.
@@ -313,11 +397,15 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned 
 // k mod gridDim.x; an entry's column is found by binary search of the prefix.
 constexpr int kFarColsMax = 8192;  // columns the prefix holds (32 KB of LDS)
 constexpr int kFarBalGrid = 1024;  // workgroups of the balanced pass (at most)
+//
+// With row lists (fr.cnt != nullptr) the pass also deposits the lists' overflow (fr.ovl, when
+// `lists` says this deposit consumed them) and clears the row counts for the next step; the overflow
+// count is cleared by the forward transform that follows (every workgroup here reads it).
 template <class Src, bool WIDE>
 __device__ __forceinline__ void deposit_far_bal_body(Src src, unsigned long long* __restrict__ mesh,
                                                      const int* __restrict__ far,
                                                      const unsigned int* __restrict__ farcnt, int ncol, int seg,
-                                                     int N, unsigned blk, unsigned nblk) {
+                                                     int N, unsigned blk, unsigned nblk, FarRows fr, int lists) {
     __shared__ unsigned int pre[kFarColsMax + 1];
     __shared__ unsigned int wsum[kThreads / 64];
     const int R = (ncol + kThreads - 1) / kThreads;  // columns per thread, contiguous
@@ -354,14 +442,19 @@ __device__ __forceinline__ void deposit_far_bal_body(Src src, unsigned long long
         }
         far_add<Src, WIDE>(src, mesh, far[(size_t)a * seg + (t - pre[a])], N);
     }
+    if (fr.cnt) {
+        const unsigned nov = lists ? min(*fr.ovcnt, fr.ovcap) : 0u;
+        for (unsigned t = blk * kThreads + threadIdx.x; t < nov; t += nblk * kThreads) far_add<Src, WIDE>(src, mesh, fr.ovl[t], N);
+        for (unsigned r = blk * kThreads + threadIdx.x; r < (unsigned)N * (unsigned)N; r += nblk * kThreads) fr.cnt[r] = 0u;
+    }
 }
 
 template <class Src, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_deposit_far_bal(Src src, unsigned long long* __restrict__ mesh,
                                                               const int* __restrict__ far,
                                                               const unsigned int* __restrict__ farcnt, int ncol,
-                                                              int seg, int N) {
-    deposit_far_bal_body<Src, WIDE>(src, mesh, far, farcnt, ncol, seg, N, blockIdx.x, gridDim.x);
+                                                              int seg, int N, FarRows fr, int lists) {
+    deposit_far_bal_body<Src, WIDE>(src, mesh, far, farcnt, ncol, seg, N, blockIdx.x, gridDim.x, fr, lists);
 }
 
 // Column shape per mesh size, cell width and LDS budget.  The candidates are the 8x8, 4x8, 4x4 and

@@ -224,11 +224,14 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R
 // Rows (x,y) of the real mesh -> half spectrum along kz.  FIXED: the input is the fixed-point
 // deposit (WIDE: 128-bit cells), converted on the fly to delta = rho/n_mean - 1 (pm3d.py:103-104),
 // with each block's maximum rho stored to bmax[blockIdx.x] (reduced by the following y pass, so no
-// two blocks contend for one address).
+// two blocks contend for one address).  `clear` (nullable): a counter the deposit just consumed
+// (the far-particle overflow count), zeroed by block 0.
 template <typename R, int N, bool FIXED, bool WIDE, int V>
 __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                     cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
-                                                    double* __restrict__ bmax, double qscale, double nmean) {
+                                                    double* __restrict__ bmax, double qscale, double nmean,
+                                                    unsigned int* __restrict__ clear) {
+    if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0u;
     constexpr int NC = N / 2, P = Plan<R, N, V>::PZ, PP = lds_pad<R>(P);
     __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
     const size_t row0 = (size_t)blockIdx.x * P;

@@ -175,13 +175,15 @@ struct PlaneLoadBatch<R, false, false> {
 // ---------------------------------------------------------------------------- forward, plane x
 // Workgroup (x, h), logical id 2x + h.  FIXED: the input is the fixed-point deposit (WIDE: 128-bit
 // cells), converted to delta = rho/n_mean - 1 on load (pm3d.py:103-104); both workgroups of a plane
-// store the plane's maximum rho to bmax[2x + h] (reduced by the x pass).
+// store the plane's maximum rho to bmax[2x + h] (reduced by the x pass).  `clear` (nullable): a
+// counter the deposit just consumed (the far-particle overflow count), zeroed by workgroup 0.
 template <typename R, int N, bool FIXED, bool WIDE>
 __device__ __forceinline__ void fwd_plane_body(const void* __restrict__ in, cplx<R>* __restrict__ spec,
                                                cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw_g,
                                                double* __restrict__ bmax,
                                                double qscale,
-                                               double nmean, XMap xm) {
+                                               double nmean, XMap xm, unsigned int* __restrict__ clear) {
+    if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0u;
     using PT = PlaneTile<R, N>;
     using LD = PlaneLoadBatch<R, FIXED, WIDE>;
     constexpr int H = PT::H, S = PT::S, NT = PT::NT;
@@ -279,8 +281,8 @@ __global__ __launch_bounds__((PlaneTile<R, N>::NT)) void k_fwd_plane(const void*
                                                                      cplx<R>* __restrict__ nyq, const cplx<R>* __restrict__ tw,
                                                                      double* __restrict__ bmax,
                                                                      double qscale,
-                                                                     double nmean, XMap xm) {
-    fwd_plane_body<R, N, FIXED, WIDE>(in, spec, nyq, tw, bmax, qscale, nmean, xm);
+                                                                     double nmean, XMap xm, unsigned int* __restrict__ clear) {
+    fwd_plane_body<R, N, FIXED, WIDE>(in, spec, nyq, tw, bmax, qscale, nmean, xm, clear);
 }
 
 // ---------------------------------------------------------------------------- inverse, plane x

@@ -171,7 +171,9 @@ __global__ __launch_bounds__(kThreads) void k_kick_drift(double* __restrict__ x,
 // with the gradient mesh of the forces at its new positions; the gather that turns it into the
 // acceleration is deferred to here, where the same positions are loaded once for the gather and
 // the drift: acc = gather3_at(x), then exactly k_kick_drift's arithmetic on it (pending closing
-// kick, opening kick, drift, each its own rounding).  The acceleration is never stored.
+// kick, opening kick, drift, each its own rounding).  The acceleration is never stored.  With row
+// lists (fr.cnt != nullptr; particles on the mesh lattice, Np == N) every particle whose new base
+// cell is not near its lattice site (the column deposit's test) is appended to its row's list.
 __global__ __launch_bounds__(kThreads) void k_gather_kick_drift(double* __restrict__ x, double* __restrict__ p,
                                                                 long long n, int N, double dx,
                                                                 const double* __restrict__ grad, int pend, double c1p,
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_kick_drift(double* __restri
                                                                 double den_d, double L,
                                                                 unsigned long long* __restrict__ bad,
                                                                 const DevSched* __restrict__ sch,
-                                                                const DevStep* __restrict__ tab, XMap xm) {
+                                                                const DevStep* __restrict__ tab, XMap xm, FarRows fr) {
     if (sch) {
         const int k = sch->counter;
         if (k >= sch->nsteps) return;
@@ -188,10 +190,11 @@ __global__ __launch_bounds__(kThreads) void k_gather_kick_drift(double* __restri
         c1p = e.c1p; denp = e.denp; c1 = e.c1; den1 = e.den1; da = e.da; den_d = e.den_d;
     }
     const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
-    bool ok = true;
+    bool ok = true, far = false;
+    unsigned row = 0;
     if (i < n) {
         const double X[3] = {x[i], x[n + i], x[2 * n + i]};
-        double a[3];
+        double a[3], xn[3];
         gather3_at(X, N, dx, grad, a);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -202,9 +205,24 @@ __global__ __launch_bounds__(kThreads) void k_gather_kick_drift(double* __restri
             const double xd = npy_rem_near(X[d] + (da * pd) / den_d, L);
             p[k] = pd;
             x[k] = xd;
+            xn[d] = xd;
             ok &= isfinite(pd) && isfinite(xd);
         }
+        if (fr.cnt) {
+            const int ii = (int)i, s[3] = {ii / (N * N), (ii / N) % N, ii % N};
+            const double inv_dx = 1.0 / dx;
+            int lo[3];
+            bool nr = true;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double f;
+                nr &= near_f64(xn[d], dx, inv_dx, N, s[d], lo[d], f);
+            }
+            far = !nr;
+            row = (unsigned)(lo[0] * N + lo[1]);
+        }
     }
+    if (fr.cnt) far_rows_append(fr, far, row, (int)i);
     const unsigned long long nb = __ballot(!ok);
     if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, (unsigned long long)__popcll(nb));
 }

@@ -165,12 +165,14 @@ __global__ __launch_bounds__(kThreads) void k_kick_drift_mx(int* __restrict__ ce
 // The gather of the previous step's forces fused with this step's kicks and drift (the mixed form
 // of k_gather_kick_drift): the cell and offset are loaded once, acc = gather3_mx_at, then exactly
 // kick_drift_mx_body's arithmetic.  The acceleration is never stored.
+// With row lists (fr.cnt != nullptr; Np == N) a particle whose new cell is not near its lattice site
+// is appended to its row's list (see k_gather_kick_drift).
 __device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell, float* __restrict__ off,
                                                           float* __restrict__ p, long long n, int N,
                                                           const float* __restrict__ grad, int pend, float kp, float k1,
                                                           float sd, unsigned long long* __restrict__ bad,
                                                           const DevSched* __restrict__ sch,
-                                                          const DevStep* __restrict__ tab, XMap xm) {
+                                                          const DevStep* __restrict__ tab, XMap xm, FarRows fr) {
     if (sch) {
         const int k = sch->counter;
         if (k >= sch->nsteps) return;
@@ -180,12 +182,14 @@ __device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell
         sd = tab[k].sd;
     }
     const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
-    bool ok = true;
+    bool ok = true, far = false;
+    unsigned row = 0;
     if (i < n) {
         int lo[3];
         float f[3], a[3];
         MxPos{cell, off, n}.load(i, lo, f);
         gather3_mx_at(lo, f, N, grad, a);
+        int nc[3] = {lo[0], lo[1], lo[2]};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const long long k = d * n + i;
@@ -210,9 +214,17 @@ __device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell
                 c %= N;
                 if (c < 0) c += N;
                 cell[k] = c;
+                nc[d] = c;
             }
         }
+        if (fr.cnt) {
+            const int ii = (int)i;
+            far = !(near_cell(nc[0], N, ii / (N * N)) && near_cell(nc[1], N, (ii / N) % N) && near_cell(nc[2], N, ii % N));
+            const bool inr = (unsigned)nc[0] < (unsigned)N && (unsigned)nc[1] < (unsigned)N;
+            row = inr ? (unsigned)(nc[0] * N + nc[1]) : 0u;
+        }
     }
+    if (fr.cnt) far_rows_append(fr, far, row, (int)i);
     const unsigned long long nb = __ballot(!ok);
     if ((threadIdx.x & 63) == 0 && nb) atomicAdd(bad, (unsigned long long)__popcll(nb));
 }
@@ -223,8 +235,8 @@ __global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx(int* __restri
                                                                    float k1, float sd,
                                                                    unsigned long long* __restrict__ bad,
                                                                    const DevSched* __restrict__ sch,
-                                                                   const DevStep* __restrict__ tab, XMap xm) {
-    gather_kick_drift_mx_body(cell, off, p, n, N, grad, pend, kp, k1, sd, bad, sch, tab, xm);
+                                                                   const DevStep* __restrict__ tab, XMap xm, FarRows fr) {
+    gather_kick_drift_mx_body(cell, off, p, n, N, grad, pend, kp, k1, sd, bad, sch, tab, xm, fr);
 }
 
 __global__ __launch_bounds__(kThreads) void k_kick_mx(float* __restrict__ p, const float* __restrict__ acc, long long n3,

@@ -85,12 +85,18 @@ struct Handle {
     void* gspec = nullptr;  // 3 x the same
     void* grad = nullptr;   // [3][M] R (component-major)
     void* tw = nullptr;     // exp(-2 pi i k/N), complex R
-    // device scalars: [0] unused, [1] bad count (sticky until pm3d_synchronize), [2..7] unused,
+    // device scalars: [0] unused, [1] bad count (sticky until pm3d_synchronize), [2] overflow count
+    // of the row lists, [3..7] unused,
     // [8 + s % kRing] max rho (double bits) of deposit serial s
     unsigned long long* dscal = nullptr;
     long long serial = 0;
-    int* far = nullptr;                // far particles, one segment per deposit column
+    int* far = nullptr;                // far particles, one segment per deposit column (or the row lists' overflow)
     unsigned int* farcnt = nullptr;    // far count per column
+    // far-particle row lists (FarRows): counts per base-cell row; the lists alias acc / accf, which
+    // only a state with acc_valid uses (a step that builds the lists clears acc_valid)
+    unsigned int* rowcnt = nullptr;
+    int row_cap = 0;
+    bool lists_fresh = false;  // the last gather-kick-drift built lists that no deposit has consumed
     double* bmax = nullptr;  // per-block maxima / minima scratch
     size_t bmax_len = 0;
     double* sbuf = nullptr;  // staging for reference-shaped entry points
@@ -207,6 +213,16 @@ int check_launch(Handle* h, const char* what) {
     return 0;
 }
 
+// Far-particle row lists of the handle (FarRows), or none (cnt == nullptr) when its deposit does
+// not use columns or uses the per-column far pass (N = 512).
+FarRows far_rows(const Handle* h) {
+    if (!h->rowcnt) return FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u};
+    return FarRows{h->rowcnt, (int*)(h->mixed ? (void*)h->accf : (void*)h->acc), h->row_cap, h->far,
+                   (unsigned int*)(h->dscal + 2), (unsigned int)h->n};
+}
+// the far-particle overflow count, cleared by the forward transform after the deposit consumed it
+unsigned int* ovf_clear(const Handle* h) { return h->rowcnt ? (unsigned int*)(h->dscal + 2) : nullptr; }
+
 // ---------------------------------------------------------------- FFT launches, R and N templated
 // Each pass runs in tile variant h->fftv[pass] (fft.hpp Plan<R, N, V>); every variant computes
 // the same bits, so the choice is made by timing (tune_fft).
@@ -234,11 +250,13 @@ struct Fft {
         const XMap xm = xmap(g, 2);  // the two workgroups of a plane read the same rows: one XCD
         Timed t(h, KC_FWD_ZY, true);
         if (fixed && h->wide)
-            t.launch(k_fwd_plane<R, N, true, true>, g, NT, in, spec, spec + specN, tw, h->bmax, h->qscale, h->nmean, xm);
+            t.launch(k_fwd_plane<R, N, true, true>, g, NT, in, spec, spec + specN, tw, h->bmax, h->qscale, h->nmean, xm,
+                     ovf_clear(h));
         else if (fixed)
-            t.launch(k_fwd_plane<R, N, true, false>, g, NT, in, spec, spec + specN, tw, h->bmax, h->qscale, h->nmean, xm);
+            t.launch(k_fwd_plane<R, N, true, false>, g, NT, in, spec, spec + specN, tw, h->bmax, h->qscale, h->nmean, xm,
+                     ovf_clear(h));
         else
-            t.launch(k_fwd_plane<R, N, false, false>, g, NT, in, spec, spec + specN, tw, nullptr, 0.0, 1.0, xm);
+            t.launch(k_fwd_plane<R, N, false, false>, g, NT, in, spec, spec + specN, tw, nullptr, 0.0, 1.0, xm, nullptr);
         h->bmax_pending = fixed ? (int)g : 0;
     }
 
@@ -276,12 +294,13 @@ struct Fft {
         Timed t(h, KC_R2C_Z);
         if (fixed && h->wide)
             k_r2c_z<R, N, true, true, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax,
-                                                                    h->qscale, h->nmean);
+                                                                    h->qscale, h->nmean, ovf_clear(h));
         else if (fixed)
             k_r2c_z<R, N, true, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, h->bmax,
-                                                                     h->qscale, h->nmean);
+                                                                     h->qscale, h->nmean, ovf_clear(h));
         else
-            k_r2c_z<R, N, false, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, 0.0, 1.0);
+            k_r2c_z<R, N, false, false, V><<<gz, kThreads, 0, h->st>>>(in, spec, spec + specN, tw, nullptr, 0.0, 1.0,
+                                                                      nullptr);
     }
 
     // nbmax: number of per-block maxima the r2c launch left in h->bmax (0 when the input was real)
@@ -549,19 +568,21 @@ unsigned far_bal_grid(long long n) {
     return (unsigned)std::min<long long>(kFarBalGrid, (n + kThreads - 1) / kThreads);
 }
 
-// Column deposit (deposit.hpp) for one mesh size and column shape variant.
+// Column deposit (deposit.hpp) for one mesh size and column shape variant; `lists`: take the far
+// particles from the row lists the step's gather-kick-drift built.
 template <int N, bool WIDE, int V, class Src>
-void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh) {
+void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh, bool lists) {
     using S = ColShape<N, WIDE, V>;
+    const FarRows fr = far_rows(h);
     {
         Timed t(h, KC_DEPOSIT, true);
         t.launch(k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024>, S::NBLK, 1024, src, mesh, h->far, h->farcnt,
-                 XMap{1, 1});
+                 lists ? fr : FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u}, XMap{1, 1});
     }
     Timed t(h, KC_DEPOSIT_FAR, true);
     if constexpr (S::NBLK <= kFarColsMax) {
         t.launch(k_deposit_far_bal<Src, WIDE>, far_bal_grid(h->n), kThreads, src, mesh, h->far, h->farcnt, S::NBLK,
-                 S::BX * S::BY * N, N);
+                 S::BX * S::BY * N, N, fr, lists ? 1 : 0);
     } else {
         // a column lists at most its own BX*BY*N sites: kFarSub workgroups per column bound the work
         // of the fullest one
@@ -571,26 +592,26 @@ void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh) {
 }
 
 template <int N, bool WIDE, class Src>
-void deposit_columns_v(Handle* h, const Src& src, unsigned long long* mesh, int v) {
+void deposit_columns_v(Handle* h, const Src& src, unsigned long long* mesh, int v, bool lists) {
     switch (v) {
-        case 1: deposit_columns<N, WIDE, 1>(h, src, mesh); break;
-        case 2: deposit_columns<N, WIDE, 2>(h, src, mesh); break;
-        case 3: deposit_columns<N, WIDE, 3>(h, src, mesh); break;
-        default: deposit_columns<N, WIDE, 0>(h, src, mesh); break;
+        case 1: deposit_columns<N, WIDE, 1>(h, src, mesh, lists); break;
+        case 2: deposit_columns<N, WIDE, 2>(h, src, mesh, lists); break;
+        case 3: deposit_columns<N, WIDE, 3>(h, src, mesh, lists); break;
+        default: deposit_columns<N, WIDE, 0>(h, src, mesh, lists); break;
     }
 }
 
 template <bool WIDE, class Src>
-int deposit_columns_n(Handle* h, const Src& src, unsigned long long* mesh, int v) {
+int deposit_columns_n(Handle* h, const Src& src, unsigned long long* mesh, int v, bool lists) {
     switch (h->N) {
-        case 16: deposit_columns_v<16, WIDE>(h, src, mesh, v); break;
-        case 32: deposit_columns_v<32, WIDE>(h, src, mesh, v); break;
-        case 48: deposit_columns_v<48, WIDE>(h, src, mesh, v); break;
-        case 64: deposit_columns_v<64, WIDE>(h, src, mesh, v); break;
-        case 96: deposit_columns_v<96, WIDE>(h, src, mesh, v); break;
-        case 128: deposit_columns_v<128, WIDE>(h, src, mesh, v); break;
-        case 256: deposit_columns_v<256, WIDE>(h, src, mesh, v); break;
-        case 512: deposit_columns_v<512, WIDE>(h, src, mesh, v); break;
+        case 16: deposit_columns_v<16, WIDE>(h, src, mesh, v, lists); break;
+        case 32: deposit_columns_v<32, WIDE>(h, src, mesh, v, lists); break;
+        case 48: deposit_columns_v<48, WIDE>(h, src, mesh, v, lists); break;
+        case 64: deposit_columns_v<64, WIDE>(h, src, mesh, v, lists); break;
+        case 96: deposit_columns_v<96, WIDE>(h, src, mesh, v, lists); break;
+        case 128: deposit_columns_v<128, WIDE>(h, src, mesh, v, lists); break;
+        case 256: deposit_columns_v<256, WIDE>(h, src, mesh, v, lists); break;
+        case 512: deposit_columns_v<512, WIDE>(h, src, mesh, v, lists); break;
         default: return fail(h, PM3D_EUNSUPPORTED, "column deposit needs N in [16,512]");
     }
     return 0;
@@ -630,8 +651,12 @@ bool columns_used(const Handle* h) { return h->Np == h->N && h->N >= 16; }
 // Deposit of the device state into h->mesh.  The column path (Np == N, N >= 16) writes every cell;
 // the atomic path clears the mesh first.  This deposit's maximum is stored into its ring slot by the
 // y pass of the FFT.
-// Deposit of the device state into `mesh` (h->mesh, or a scratch mesh of the same size).
-int deposit_into(Handle* h, unsigned long long* mesh) {
+// Deposit of the device state into `mesh` (h->mesh, or a scratch mesh of the same size).  `lists`:
+// the far particles are in the row lists of the step's gather-kick-drift (only the step's own
+// deposit of the state).  Every column deposit's far pass clears the row counts, so lists not
+// consumed by this deposit are gone after it.
+int deposit_into(Handle* h, unsigned long long* mesh, bool lists = false) {
+    h->lists_fresh = false;
     const bool tiled = columns_used(h);  // a column plus its one-site ring must not wrap onto itself
     const int W = h->wide ? 2 : 1;
     if (!tiled) {
@@ -641,7 +666,7 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
     int rc = 0;
     if (!h->mixed) {
         if (tiled) {
-            rc = deposit_columns_n<true>(h, src_f64(h), mesh, h->depv);
+            rc = deposit_columns_n<true>(h, src_f64(h), mesh, h->depv, lists);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic<true>
@@ -649,7 +674,7 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
         }
     } else {
         if (tiled) {
-            rc = deposit_columns_n<false>(h, src_mx(h), mesh, h->depv);
+            rc = deposit_columns_n<false>(h, src_mx(h), mesh, h->depv, lists);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, mesh);
@@ -660,7 +685,7 @@ int deposit_into(Handle* h, unsigned long long* mesh) {
 
 int deposit_state(Handle* h) {
     h->serial += 1;
-    return deposit_into(h, h->mesh);
+    return deposit_into(h, h->mesh, h->lists_fresh);
 }
 
 // Picks the column shape of the deposit (ColShape variants: 8x8, 4x8, 4x4, 2x4 columns, those that
@@ -700,8 +725,8 @@ int tune_deposit(Handle* h) {
         return fail(h, PM3D_EHIP, "hipEventCreate failed");
     }
     auto launch = [&](int v) {
-        return h->mixed ? deposit_columns_n<false>(h, src_mx(h), h->mesh, v)
-                        : deposit_columns_n<true>(h, src_f64(h), h->mesh, v);
+        return h->mixed ? deposit_columns_n<false>(h, src_mx(h), h->mesh, v, false)
+                        : deposit_columns_n<true>(h, src_f64(h), h->mesh, v, false);
     };
     // distinct shapes (variants past the last fitting shape repeat it)
     int nv = 0;
@@ -817,7 +842,7 @@ int read_bad(Handle* h, int* bad) {
 
 void free_all(Handle* h) {
     void* bufs[] = {h->x, h->p, h->acc, h->cell, h->off, h->pf, h->accf, h->mesh, h->spec, h->gspec,
-                    h->grad, h->tw, h->dscal, h->far, h->farcnt, h->bmax, h->sbuf, h->sched, h->tab, h->jhist,
+                    h->grad, h->tw, h->dscal, h->far, h->farcnt, h->rowcnt, h->bmax, h->sbuf, h->sched, h->tab, h->jhist,
                     h->an_idx, h->an_off, h->an_buf, h->an_cnt};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -940,14 +965,16 @@ void spec_amp_n(Handle* h, double kcut) {
 // the kicks and drift of the scheduled scalars, force meshes, Jacobian minimum, schedule advance.
 // Launches only; no host synchronisation.
 int launch_sched_step(Handle* h) {
+    const FarRows fr = far_rows(h);
     if (!h->mixed)
         k_gather_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->n, h->N, h->dx, (const double*)h->grad,
                                                                     1, 0, 1, 0, 1, 0, 1, h->L, h->dscal + 1, h->sched,
-                                                                    h->tab, gather_map(h));
+                                                                    h->tab, gather_map(h), fr);
     else
         k_gather_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(h->cell, h->off, h->pf, h->n, h->N,
                                                                        (const float*)h->grad, 1, 0, 0, 0, h->dscal + 1,
-                                                                       h->sched, h->tab, gather_map(h));
+                                                                       h->sched, h->tab, gather_map(h), fr);
+    h->lists_fresh = fr.cnt != nullptr;
     h->use_sched = true;
     int rc = solve_state(h, 0.0, 0.0);
     h->use_sched = false;
@@ -1004,6 +1031,7 @@ MxBox box_of(Handle* h) {
     b.grad = (float*)h->grad;
     b.tw = (const cplx<float>*)h->tw;
     b.bmax = h->bmax;
+    b.fr = far_rows(h);
     b.qslot = qmax_slot(h);
     b.kval = 1.0 / ((double)h->N * h->dx);
     b.inv_c3 = inv_c3(h);
@@ -1020,6 +1048,7 @@ void launch_boxes_deposit(Handle* h0, const MxBox* dev, int nb, hipStream_t st) 
     k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, XMap{1, 1});
     if constexpr (S::NBLK <= kFarColsMax)
         k_deposit_far_bal_boxes<<<dim3(far_bal_grid(h0->n), nb), kThreads, 0, st>>>(dev, S::NBLK, S::BX * S::BY * N);
+    // (the grouped gather-kick-drift of the same replay built every member's row lists)
     else
         k_deposit_far_seg_boxes<<<dim3(S::NBLK, kFarSub, nb), kThreads, 0, st>>>(dev, S::BX * S::BY * N);
 }
@@ -1236,6 +1265,26 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         hipMemset(h->farcnt, 0, (size_t)N * N * sizeof(unsigned int)) != hipSuccess) {
         h->err = "initial copies failed";
         return bail(PM3D_EHIP);
+    }
+    // far-particle row lists: column deposit with the balanced far pass (N <= 256); capacity 2N per
+    // row.  PM3D_ROW_CAP = 1 .. 2N overrides it (to exercise the overflow path), 0 turns the lists
+    // off (every far particle through the global-atomic pass); all give the same mesh bits.
+    int cap = (columns_used(h) && N <= 256) ? 2 * N : 0;
+    if (const char* env = std::getenv("PM3D_ROW_CAP")) {
+        const int c = std::atoi(env);
+        if (c < 0 || c > 2 * N) {
+            h->err = "PM3D_ROW_CAP must be in [0, 2N]";
+            return bail(PM3D_EINVAL);
+        }
+        if (cap) cap = c;
+    }
+    if (cap) {
+        h->row_cap = cap;
+        if ((rc = dalloc(h, &h->rowcnt, (size_t)N * N))) return bail(rc);
+        if (hipMemset(h->rowcnt, 0, (size_t)N * N * sizeof(unsigned int)) != hipSuccess) {
+            h->err = "initial copies failed";
+            return bail(PM3D_EHIP);
+        }
     }
     if ((rc = tune_fft(h)) || (rc = tune_deposit(h))) return bail(rc);
     // the timing launches wrote a deposit maximum into the ring
@@ -1521,15 +1570,17 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
         // pending closing kick of the previous step (:127) + opening kick (:122) + drift
         // (:123-124) in one pass; FP64 keeps the reference's two separate roundings of p
         Timed t(h, KC_GATHER_KICK, true);
+        const FarRows fr = far_rows(h);
         if (!h->mixed)
             t.launch(k_gather_kick_drift, grid_for(h->n), kThreads, h->x, h->p, h->n, h->N, h->dx,
                      (const double*)h->grad, h->kick_pending ? 1 : 0, h->pend_c1, h->pend_den, sc->c1, sc->den1, sc->da,
-                     sc->den_d, h->L, h->dscal + 1, nullptr, nullptr, gather_map(h));
+                     sc->den_d, h->L, h->dscal + 1, nullptr, nullptr, gather_map(h), fr);
         else
             t.launch(k_gather_kick_drift_mx, grid_for(h->n), kThreads, h->cell, h->off, h->pf, h->n, h->N,
                      (const float*)h->grad, h->kick_pending ? 1 : 0,
                      (float)(h->kick_pending ? h->pend_c1 / h->pend_den : 0.0), (float)(sc->c1 / sc->den1),
-                     (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr, gather_map(h));
+                     (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr, gather_map(h), fr);
+        h->lists_fresh = fr.cnt != nullptr;
     } else {
         // the stored acceleration (a state whose force meshes were replaced after it was gathered)
         Timed t(h, KC_KICK_DRIFT);
