@@ -161,6 +161,13 @@ int pm3d_analyse(void* handle, const double* radii, int nradii, double* centre, 
  * update pm3d.py:122,127) and has no such diagnostic. */
 int pm3d_totals(void* handle, double* mass, double* mass_dev, double* momentum);
 
+/* Attainable HBM bandwidth of this GPU, for reading kernel rooflines against: a hand-written
+ * streaming copy (16-byte non-temporal vector loads, then stores, one pass; read + write bytes
+ * counted) and a streaming read of `bytes` (1 MiB .. 64 GiB), each launched `reps` times on
+ * buffers the call allocates and frees.
+ * No reference counterpart (a diagnostic of this library). */
+int pm3d_stream_probe(int64_t bytes, int reps, double* copy_gbs, double* read_gbs);
+
 /* FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z), two bits: bit 0 = pencils per
  * workgroup sized for 32 KB rather than 16 KB LDS buffers, bit 1 = one buffer transformed in place
  * rather than two ping-pong buffers.  Every variant gives the same bits; pm3d_create times all four

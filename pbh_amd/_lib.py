@@ -71,6 +71,7 @@ SIGNATURES = {
     "pm3d_get_timings": (c_int, [c_void_p, PI64, PD, c_int]),
     "pm3d_kernel_names": (c_char_p, []),
     "pm3d_totals": (c_int, [c_void_p, PD, PD, PD]),
+    "pm3d_stream_probe": (c_int, [c_int64, c_int, PD, PD]),
     "pm3d_fft_tiles": (c_int, [c_void_p, POINTER(c_int32), c_int]),
     "pm3d_deposit_tiles": (c_int, [c_void_p, POINTER(c_int32)]),
     "pm3d_device_info": (c_int, [c_void_p, c_char_p, c_int, c_char_p, c_int, PI64]),

@@ -994,6 +994,8 @@ struct Batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> joins;
+    std::vector<hipStream_t> cls_st;   // one capture stream per mesh-size class (graph branches)
+    std::vector<hipEvent_t> cls_join;
     std::vector<Handle*> members;
     std::vector<MxBox*> dbox;  // device descriptor arrays of the box-grouped classes
     std::string err;
@@ -1168,6 +1170,37 @@ int scratch_on_device(unsigned long long** out) {
     }
     *out = it->second;
     return 0;
+}
+
+// ------------------------------------------------------------ attainable-bandwidth probe
+// Streaming kernels of the probe: one pass over the buffer, every thread moving U 16-byte vectors
+// (all loads issued before the stores) with non-temporal loads and stores.  Measured on MI355X
+// (1 GiB copy, GB/s): grid-stride 8192 workgroups 5540 (U = 4), one pass 5860, one pass
+// non-temporal 6360; U = 8 and U = 1 were slower.
+typedef unsigned int probe_v4 __attribute__((ext_vector_type(4)));
+constexpr int kProbeU = 4;
+__global__ __launch_bounds__(kThreads) void k_probe_copy(const probe_v4* __restrict__ src, probe_v4* __restrict__ dst,
+                                                         long long n16) {
+    const long long b = (long long)blockIdx.x * kThreads * kProbeU + threadIdx.x;
+    probe_v4 v[kProbeU];
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u)
+        if (b + u * kThreads < n16) v[u] = __builtin_nontemporal_load(src + b + u * kThreads);
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u)
+        if (b + u * kThreads < n16) __builtin_nontemporal_store(v[u], dst + b + u * kThreads);
+}
+__global__ __launch_bounds__(kThreads) void k_probe_read(const probe_v4* __restrict__ src, long long n16,
+                                                         unsigned* __restrict__ out) {
+    const long long b = (long long)blockIdx.x * kThreads * kProbeU + threadIdx.x;
+    unsigned acc = 0;
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u)
+        if (b + u * kThreads < n16) {
+            const probe_v4 v = __builtin_nontemporal_load(src + b + u * kThreads);
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    if (acc == 0x9e3779b9u) out[blockIdx.x & 8191] = acc;  // keeps the loads; practically never stores
 }
 
 }  // namespace
@@ -1805,6 +1838,57 @@ int pm3d_totals(void* handle, double* mass, double* mass_dev, double* momentum) 
     return 0;
 }
 
+int pm3d_stream_probe(int64_t bytes, int reps, double* copy_gbs, double* read_gbs) {
+    if (bytes < (1 << 20) || bytes > (int64_t)1 << 36 || reps < 1 || !copy_gbs || !read_gbs) {
+        g_create_error = "pm3d_stream_probe: bytes in [1 MiB, 64 GiB], reps >= 1";
+        return PM3D_EINVAL;
+    }
+    const long long n16 = bytes / 16;
+    probe_v4 *a = nullptr, *b = nullptr;
+    unsigned* sink = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    const unsigned grid = (unsigned)((n16 + kThreads * kProbeU - 1) / (kThreads * kProbeU));  // one pass
+    float ms_copy = 0.f, ms_read = 0.f;
+    auto ok = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && !rc) {
+            g_create_error = std::string("pm3d_stream_probe: ") + what + ": " + hipGetErrorString(e);
+            rc = PM3D_EHIP;
+        }
+        return !rc;
+    };
+    if (ok(hipMalloc(&a, (size_t)n16 * 16), "hipMalloc") && ok(hipMalloc(&b, (size_t)n16 * 16), "hipMalloc") &&
+        ok(hipMalloc(&sink, 8192 * sizeof(unsigned)), "hipMalloc") &&
+        ok(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream") && ok(hipEventCreate(&e0), "event") &&
+        ok(hipEventCreate(&e1), "event") && ok(hipMemsetAsync(a, 1, (size_t)n16 * 16, st), "memset")) {
+        k_probe_copy<<<grid, kThreads, 0, st>>>(a, b, n16);  // warm-up
+        k_probe_read<<<grid, kThreads, 0, st>>>(a, n16, sink);
+        ok(hipEventRecord(e0, st), "record");
+        for (int r = 0; r < reps; ++r) k_probe_copy<<<grid, kThreads, 0, st>>>(a, b, n16);
+        ok(hipEventRecord(e1, st), "record");
+        ok(hipEventSynchronize(e1), "sync");
+        ok(hipEventElapsedTime(&ms_copy, e0, e1), "elapsed");
+        ok(hipEventRecord(e0, st), "record");
+        for (int r = 0; r < reps; ++r) k_probe_read<<<grid, kThreads, 0, st>>>(a, n16, sink);
+        ok(hipEventRecord(e1, st), "record");
+        ok(hipEventSynchronize(e1), "sync");
+        ok(hipEventElapsedTime(&ms_read, e0, e1), "elapsed");
+        ok(hipGetLastError(), "launch");
+    }
+    if (st) hipStreamSynchronize(st);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (sink) hipFree(sink);
+    if (rc) return rc;
+    *copy_gbs = 2.0 * (double)n16 * 16 * reps / (ms_copy * 1e-3) / 1e9;
+    *read_gbs = (double)n16 * 16 * reps / (ms_read * 1e-3) / 1e9;
+    return 0;
+}
+
 int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax) {
     if (!handle || !variants || nmax < 0) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
@@ -1994,6 +2078,8 @@ int pm3d_batch_destroy(void* batch) {
     batch_clear_graph(b);
     batch_free_boxes(b);
     for (auto e : b->joins) hipEventDestroy(e);
+    for (auto e : b->cls_join) hipEventDestroy(e);
+    for (auto cs : b->cls_st) hipStreamDestroy(cs);
     hipEventDestroy(b->fork);
     hipStreamDestroy(b->st);
     delete b;
@@ -2035,6 +2121,20 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
     std::vector<bool> grouped(count, false);
     for (int i = 0; i < count; ++i)
         if (box_groupable(b->members[i])) classes[b->members[i]->N].push_back(i);
+    while (b->cls_st.size() < classes.size()) {
+        hipStream_t cs;
+        hipEvent_t ce;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+            b->err = "stream creation failed";
+            return PM3D_EHIP;
+        }
+        b->cls_st.push_back(cs);
+        if (hipEventCreateWithFlags(&ce, hipEventDisableTiming) != hipSuccess) {
+            b->err = "event creation failed";
+            return PM3D_EHIP;
+        }
+        b->cls_join.push_back(ce);
+    }
     std::vector<std::pair<int, MxBox*>> cls_dev;  // (first member, descriptors)
     for (auto& kv : classes) {
         std::vector<MxBox> hb;
@@ -2063,10 +2163,23 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
     }
     hipError_t e = hipEventRecord(b->fork, b->st);
     int rc = 0;
+    // each class is its own branch of the graph, so its kernels may run beside another class's
+    // (64-peak set: 201 -> 209 solves/s); PM3D_BATCH_BRANCHES=0 runs the classes one after the other
+    const char* br = std::getenv("PM3D_BATCH_BRANCHES");
+    const bool branches = !(br && br[0] == '0');
     for (size_t c = 0; c < cls_dev.size() && e == hipSuccess; ++c) {
         Handle* h0 = b->members[cls_dev[c].first];
-        launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), b->st);
-        e = hipGetLastError();
+        if (branches) {
+            hipStream_t cs = b->cls_st[c];
+            e = hipStreamWaitEvent(cs, b->fork, 0);
+            if (e == hipSuccess) launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), cs);
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e == hipSuccess) e = hipEventRecord(b->cls_join[c], cs);
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->st, b->cls_join[c], 0);
+        } else {
+            launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), b->st);
+            e = hipGetLastError();
+        }
     }
     for (int i = 0; i < count && e == hipSuccess && rc == 0; ++i) {
         if (grouped[i]) continue;

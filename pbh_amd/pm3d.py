@@ -436,6 +436,15 @@ class PM3D:
         return {nm: (int(c), float(t)) for nm, c, t in zip(names, cnt, ms)}
 
 
+def stream_probe(nbytes: int = 1 << 30, reps: int = 10) -> dict:
+    """Attainable HBM bandwidth of the current GPU (GB/s): a hand-written 16-byte-vector streaming
+    copy (read + write bytes) and a streaming read over `nbytes` (pm3d_stream_probe)."""
+    L = _lib.load()
+    cp, rd = ctypes.c_double(), ctypes.c_double()
+    check(L.pm3d_stream_probe(int(nbytes), int(reps), ctypes.byref(cp), ctypes.byref(rd)), None)
+    return {"copy_gbs": cp.value, "read_gbs": rd.value, "bytes": int(nbytes), "reps": int(reps)}
+
+
 def analyse(pm: PM3D, a: float, fracs=(0.1, 0.2, 0.3), radii_cells=(1.0, 2.0, 4.0, 8.0)):
     """Inner Lagrangian-set axes and enclosed-mass compaction (pm3d.py:132-159), on the device.
 
