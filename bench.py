@@ -354,6 +354,7 @@ def main():
                    "note": "after the crossing step; exact fixed-point mass total and FP64 momentum sum"},
         "fft_tiles": run.pm.fft_tiles(),
         "deposit_tiles": list(run.pm.deposit_tiles()),
+        "deposit_trials": run.pm.deposit_trials(),
     }
     run.pm.close()
     from pbh_amd.pm3d import stream_probe

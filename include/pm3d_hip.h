@@ -176,14 +176,23 @@ int pm3d_stream_probe(int64_t bytes, int reps, double* copy_gbs, double* read_gb
  * passes.  No reference counterpart (numpy's FFT has no tiles). */
 int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax);
 
-/* Column shape of the deposit: shape[0..2] = BX, BY, N of the owner-computes columns (BX x BY x N
- * cells per workgroup), all 0 for the atomic path (Np != N or N < 16).  pm3d_create times the 8x8,
- * 4x8, 4x4 and 2x4 columns that fit in LDS on a jittered lattice and keeps the widest unless a
- * narrower one is at least 3 % faster (cached per mesh size and precision); every shape gives the
- * same mesh bits.  Environment PM3D_DEPOSIT_SHAPE =
- * one digit 0-3 forces the n-th fitting shape.  Returns 1 for columns, 0 for the atomic path.  No
- * reference counterpart (pm3d.py:81-89 deposits with one np.bincount per CIC corner). */
+/* Current column shape of the deposit: shape[0..2] = BX, BY, N of the owner-computes columns
+ * (BX x BY x N cells per workgroup), all 0 for the atomic path (Np != N or N < 16).  pm3d_create
+ * times the 8x8, 4x8, 4x4 and 2x4 columns that fit in LDS on a jittered lattice and keeps the
+ * widest unless a narrower one is at least 3 % faster (cached per mesh size and precision); the
+ * shapes within 25 % of the fastest stay candidates, and every 64 step deposits pm3d_step times two
+ * deposits in each (event pairs read back without waiting) and moves to one that is 2 % faster than
+ * the current shape, as clustering changes which is best.  Every shape gives the same mesh bits.
+ * Environment PM3D_DEPOSIT_SHAPE = auto (default), static (the creation choice throughout), all
+ * (every fitting shape a trial candidate), or one digit 0-3 forcing the n-th fitting shape.
+ * Returns 1 for columns, 0 for the atomic path.  No reference counterpart (pm3d.py:81-89 deposits
+ * with one np.bincount per CIC corner). */
 int pm3d_deposit_tiles(void* handle, int32_t* shape);
+
+/* Shape trials of the step deposits so far: out[0] = finished trials, out[1] = shape changes they
+ * made, out[2] = candidate mask (bit v: shape variant v; fewer than two bits: no trials).  No
+ * reference counterpart (diagnostic of pm3d_deposit_tiles' trials). */
+int pm3d_deposit_trials(void* handle, int64_t* out);
 
 /* Initial conditions built on the device (the construction of PM3D.make_ic, pm3d.py:59-68,
  * generalised to an explicit linear density field at a_i): delta = mu*exp(-0.5*sum (x_c/s_c)^2)

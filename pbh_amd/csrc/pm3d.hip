@@ -65,6 +65,25 @@ enum KernelClass {
 const char* kKernelNames =
     "kick_drift,fill,deposit,deposit_far,r2c_z,y_fwd,x_poisson,y_inv,c2r_z,gather,other,fwd_zy,inv_yz,gather_kick";
 
+// Trials of the step deposit's column shape during a run (live_deposit).  Every column shape writes
+// the same mesh bits, so any deposit may use any of them; as particles cluster, the workgroups of
+// the dense columns take longer and a narrower shape (more, smaller workgroups) can overtake the
+// one chosen at creation on a lattice.
+constexpr int kLivePeriod = 64;  // step deposits between the starts of two trials
+constexpr int kLiveReps = 2;     // deposits per candidate in a trial (its faster one counts)
+constexpr int kLiveSlots = 4 * kLiveReps;
+struct LiveShape {
+    unsigned cand = 0;       // bit v: shape variant v is a candidate (0 or one bit: no trials)
+    long long deposits = 0;  // step deposits made
+    long long next = 0;      // step deposit at which the next trial starts (a new state: its first)
+    int plan[kLiveSlots] = {};  // variants of the running trial, in launch order (the current one last)
+    int nplan = 0, pos = 0;     // trial length (0: none running), deposits of it launched
+    int unread = 0;             // samples of the trial not yet read
+    float best[4] = {};         // fastest sample per variant in the running trial (ms)
+    hipEvent_t ev[2 * kLiveSlots] = {};  // event pair of each sample, in plan order
+    int trials = 0, changes = 0;         // finished trials, shape changes
+};
+
 struct Handle {
     pm3d_cfg cfg{};
     int N = 0, Np = 0, log2N = 0;
@@ -123,6 +142,8 @@ struct Handle {
     size_t bytes = 0;
     int fftv[5] = {};  // FFT tile variant per pass (r2c_z, y fwd, x, y inv, c2r_z)
     int depv = 0;      // column shape variant of the deposit (ColShape<N, WIDE, V>)
+    LiveShape live;    // trials of the step deposit's column shape during a run
+    bool live_arm = false;  // the next deposit_state is a step's own deposit (pm3d_step)
     int bmax_pending = 0;  // per-block deposit maxima left in bmax by the plane forward pass
     // analysis sets (pm3d_set_analysis_sets): indices, offsets, centre/moment/partial buffers
     int* an_idx = nullptr;
@@ -655,8 +676,10 @@ bool columns_used(const Handle* h) { return h->Np == h->N && h->N >= 16; }
 // the far particles are in the row lists of the step's gather-kick-drift (only the step's own
 // deposit of the state).  Every column deposit's far pass clears the row counts, so lists not
 // consumed by this deposit are gone after it.
-int deposit_into(Handle* h, unsigned long long* mesh, bool lists = false) {
+// `v`: column shape variant (h->depv when negative).
+int deposit_into(Handle* h, unsigned long long* mesh, bool lists = false, int v = -1) {
     h->lists_fresh = false;
+    if (v < 0) v = h->depv;
     const bool tiled = columns_used(h);  // a column plus its one-site ring must not wrap onto itself
     const int W = h->wide ? 2 : 1;
     if (!tiled) {
@@ -666,7 +689,7 @@ int deposit_into(Handle* h, unsigned long long* mesh, bool lists = false) {
     int rc = 0;
     if (!h->mixed) {
         if (tiled) {
-            rc = deposit_columns_n<true>(h, src_f64(h), mesh, h->depv, lists);
+            rc = deposit_columns_n<true>(h, src_f64(h), mesh, v, lists);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic<true>
@@ -674,7 +697,7 @@ int deposit_into(Handle* h, unsigned long long* mesh, bool lists = false) {
         }
     } else {
         if (tiled) {
-            rc = deposit_columns_n<false>(h, src_mx(h), mesh, h->depv, lists);
+            rc = deposit_columns_n<false>(h, src_mx(h), mesh, v, lists);
         } else {
             Timed t(h, KC_DEPOSIT);
             k_deposit_atomic_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, mesh);
@@ -683,8 +706,65 @@ int deposit_into(Handle* h, unsigned long long* mesh, bool lists = false) {
     return rc ? rc : check_launch(h, "deposit");
 }
 
+// Reads the finished samples of the running trial (in launch order, without waiting) and, once all
+// are in, takes the fastest candidate when it beats the current shape by 2 %.
+void live_read(Handle* h) {
+    LiveShape& t = h->live;
+    while (t.unread > 0) {
+        const int s = t.pos - t.unread;
+        const hipError_t q = hipEventQuery(t.ev[2 * s + 1]);
+        if (q != hipSuccess) {
+            if (q == hipErrorNotReady) (void)hipGetLastError();  // not an error: leave no trace of it
+            return;
+        }
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.ev[2 * s], t.ev[2 * s + 1]) == hipSuccess && ms < t.best[t.plan[s]])
+            t.best[t.plan[s]] = ms;
+        --t.unread;
+    }
+    if (t.nplan && t.pos == t.nplan) {
+        int best = h->depv;
+        for (int v = 0; v < 4; ++v)
+            if ((t.cand >> v & 1u) && t.best[v] < 0.98f * t.best[h->depv] && t.best[v] < t.best[best]) best = v;
+        t.changes += best != h->depv;
+        h->depv = best;
+        t.nplan = t.pos = 0;
+        ++t.trials;
+    }
+}
+
+// A step's own deposit of the state.  Every kLivePeriod of them (and at the first after a new state
+// is set) a trial starts: kLiveReps deposits
+// in each candidate shape, the current one last, each between an event pair on the handle's
+// stream; live_read takes the result at later steps.
+int live_deposit(Handle* h) {
+    LiveShape& t = h->live;
+    live_read(h);
+    if (!t.nplan && t.deposits >= t.next) {
+        t.next = t.deposits + kLivePeriod;
+        for (int v = 0; v < 4; ++v) {
+            t.best[v] = 1e30f;
+            if ((t.cand >> v & 1u) && v != h->depv)
+                for (int r = 0; r < kLiveReps; ++r) t.plan[t.nplan++] = v;
+        }
+        for (int r = 0; r < kLiveReps; ++r) t.plan[t.nplan++] = h->depv;
+        t.pos = t.unread = 0;
+    }
+    ++t.deposits;
+    if (!t.nplan || t.pos == t.nplan) return deposit_into(h, h->mesh, h->lists_fresh);
+    const int s = t.pos;
+    if (hipEventRecord(t.ev[2 * s], h->st) != hipSuccess) return fail(h, PM3D_EHIP, "hipEventRecord failed");
+    int rc = deposit_into(h, h->mesh, h->lists_fresh, t.plan[s]);
+    if (rc) return rc;
+    if (hipEventRecord(t.ev[2 * s + 1], h->st) != hipSuccess) return fail(h, PM3D_EHIP, "hipEventRecord failed");
+    ++t.pos;
+    ++t.unread;
+    return 0;
+}
+
 int deposit_state(Handle* h) {
     h->serial += 1;
+    if (h->live_arm && __builtin_popcount(h->live.cand) > 1 && !h->use_sched) return live_deposit(h);
     return deposit_into(h, h->mesh, h->lists_fresh);
 }
 
@@ -692,27 +772,59 @@ int deposit_state(Handle* h) {
 // fit in LDS) by timing the column and far passes of each (3 interleaved rounds of 4 launches) on a
 // jittered lattice written into this handle's position buffers (k_tune_lattice; pm3d_set_state or
 // pm3d_make_peak_ic replaces it).  Every shape gives the same mesh bits, so the choice never changes
-// a result.  Cached per (N, precision) for the process.  PM3D_DEPOSIT_SHAPE = one digit 0-3 forces
-// the variant.
-std::map<std::pair<int, bool>, int> g_dep_tuned;
+// a result.  The shapes within 25 % of the fastest stay candidates of the trials during a run
+// (live_deposit).  Cached per (N, precision) for the process.  PM3D_DEPOSIT_SHAPE: "auto" (default),
+// "static" (no trials during a run), "all" (every fitting shape a candidate), or one digit 0-3
+// forcing the variant.
+struct DepTuned {
+    int v;
+    unsigned cand;
+};
+std::map<std::pair<int, bool>, DepTuned> g_dep_tuned;
+
+// Event pairs of the live trials, when there are at least two candidates.
+int live_init(Handle* h, unsigned cand) {
+    h->live.cand = cand;
+    if (__builtin_popcount(cand) < 2) return 0;
+    for (auto& e : h->live.ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            h->live.cand = 0;
+            return fail(h, PM3D_EHIP, "hipEventCreate failed");
+        }
+    return 0;
+}
 
 int tune_deposit(Handle* h) {
     h->depv = 0;
     if (!columns_used(h)) return 0;
     const char* env = std::getenv("PM3D_DEPOSIT_SHAPE");
-    if (env && *env && std::strcmp(env, "auto") != 0) {
+    const bool live = !(env && std::strcmp(env, "static") == 0);
+    const bool all = env && std::strcmp(env, "all") == 0;
+    if (env && *env && std::strcmp(env, "auto") != 0 && live && !all) {
         if (std::strlen(env) != 1 || env[0] < '0' || env[0] > '3')
-            return fail(h, PM3D_EINVAL, "PM3D_DEPOSIT_SHAPE must be auto or one digit 0-3");
+            return fail(h, PM3D_EINVAL, "PM3D_DEPOSIT_SHAPE must be auto, static, all or one digit 0-3");
         h->depv = env[0] - '0';
         return 0;
     }
+    // distinct shapes (variants past the last fitting shape repeat it)
+    int nv = 0;
+    for (int v = 0, pbx = -1, pby = -1; v < 4; ++v) {
+        int bx, by;
+        col_shape(h, v, bx, by);
+        if (bx == pbx && by == pby) break;
+        pbx = bx, pby = by, nv = v + 1;
+    }
+    const unsigned every = (1u << nv) - 1u;
     const std::pair<int, bool> key{h->N, h->mixed};
     {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
+        std::unique_lock<std::mutex> lk(g_tune_mu);
         auto it = g_dep_tuned.find(key);
         if (it != g_dep_tuned.end()) {
-            h->depv = it->second;
-            return 0;
+            const DepTuned t = it->second;
+            lk.unlock();
+            h->depv = t.v;
+            return live_init(h, all ? every : live ? t.cand : 0u);
         }
     }
     k_tune_lattice<<<grid_for(h->n), kThreads, 0, h->st>>>(h->N, h->dx, h->mixed ? nullptr : h->x, h->cell, h->off);
@@ -728,14 +840,6 @@ int tune_deposit(Handle* h) {
         return h->mixed ? deposit_columns_n<false>(h, src_mx(h), h->mesh, v, false)
                         : deposit_columns_n<true>(h, src_f64(h), h->mesh, v, false);
     };
-    // distinct shapes (variants past the last fitting shape repeat it)
-    int nv = 0;
-    for (int v = 0, pbx = -1, pby = -1; v < 4; ++v) {
-        int bx, by;
-        col_shape(h, v, bx, by);
-        if (bx == pbx && by == pby) break;
-        pbx = bx, pby = by, nv = v + 1;
-    }
     // three interleaved rounds of 4 launches per shape (after one warm-up each); a shape's time is
     // its fastest round, and a narrower shape replaces the widest only when 3 % faster
     float best_ms[4] = {1e30f, 1e30f, 1e30f, 1e30f};
@@ -759,10 +863,15 @@ int tune_deposit(Handle* h) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc) return rc;
+    unsigned cand = 0;
+    for (int v = 0; v < nv; ++v)
+        if (best_ms[v] <= 1.25f * best_ms[best]) cand |= 1u << v;
     h->depv = best;
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_dep_tuned[key] = best;
-    return 0;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_dep_tuned[key] = DepTuned{best, cand};
+    }
+    return live_init(h, all ? every : live ? cand : 0u);
 }
 
 // 1/c^3 for the mixed Jacobian, c = N/Np cells per lattice site
@@ -851,6 +960,8 @@ void free_all(Handle* h) {
         hipEventDestroy(u.second.second);
     }
     for (auto e : h->ev_pool) hipEventDestroy(e);
+    for (auto e : h->live.ev)
+        if (e) hipEventDestroy(e);
     if (h->st) hipStreamDestroy(h->st);
 }
 
@@ -1401,6 +1512,7 @@ int pm3d_set_state(void* handle, const double* x, const double* p, int64_t n) {
     }
     HCHK(h, hipStreamSynchronize(h->st));
     h->state_set = true;
+    h->live.next = h->live.deposits;  // a trial at the new state's first step
     return check_launch(h, "set_state");
 }
 
@@ -1633,7 +1745,10 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
     if ((rc = check_launch(h, "kick_drift"))) return rc;
     // the gather and the closing kick are deferred: the force meshes stay in h->grad, and the next
     // step gathers them with its kicks (or they are gathered and the kick applied before p is read)
-    if ((rc = solve_state(h, sc->s_new, sc->sm2))) return rc;
+    h->live_arm = true;
+    rc = solve_state(h, sc->s_new, sc->sm2);
+    h->live_arm = false;
+    if (rc) return rc;
     h->grad_valid = true;
     h->kick_pending = true;
     h->pend_c1 = sc->c1;
@@ -1909,6 +2024,15 @@ int pm3d_deposit_tiles(void* handle, int32_t* shape) {
     return 1;
 }
 
+int pm3d_deposit_trials(void* handle, int64_t* out) {
+    if (!handle || !out) return PM3D_EINVAL;
+    const LiveShape& t = ((Handle*)handle)->live;
+    out[0] = t.trials;
+    out[1] = t.changes;
+    out[2] = t.cand;
+    return 0;
+}
+
 int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic) {
     if (!handle || !ic) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
@@ -1956,6 +2080,7 @@ int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic) {
     k_lattice<<<grid_for(h->n), kThreads, 0, h->st>>>((const double*)h->grad, h->N, h->Np, h->L, ic->coef, h->x, h->p);
     HCHK(h, hipStreamSynchronize(h->st));
     h->state_set = true;
+    h->live.next = h->live.deposits;  // a trial at the new state's first step
     h->acc_valid = false;
     h->grad_valid = false;
     h->kick_pending = false;
@@ -1978,6 +2103,7 @@ int pm3d_copy_state(void* dst_handle, void* src_handle) {
     }
     HCHK(d, hipStreamSynchronize(d->st));
     d->state_set = true;
+    d->live.next = d->live.deposits;  // a trial at the new state's first step
     d->acc_valid = false;
     d->grad_valid = false;
     d->kick_pending = false;

@@ -423,10 +423,17 @@ class PM3D:
         return dict(zip(("r2c_z", "y_fwd", "x", "y_inv", "c2r_z"), (int(t) for t in v)))
 
     def deposit_tiles(self) -> tuple:
-        """Deposit column shape (BX, BY, N) chosen at creation; (0, 0, 0) for the atomic deposit."""
+        """Current deposit column shape (BX, BY, N); (0, 0, 0) for the atomic deposit.  Chosen at
+        creation, then re-chosen by timed trials every 64 steps (every shape gives the same bits)."""
         v = np.zeros(3, np.int32)
         check(self._L.pm3d_deposit_tiles(self._h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), self._h)
         return tuple(int(t) for t in v)
+
+    def deposit_trials(self) -> dict:
+        """Shape trials of the step deposits so far: finished trials, shape changes, candidate mask."""
+        v = np.zeros(3, np.int64)
+        check(self._L.pm3d_deposit_trials(self._h, iptr(v)), self._h)
+        return {"trials": int(v[0]), "changes": int(v[1]), "candidates": int(v[2])}
 
     def timings(self) -> dict:
         names = self._L.pm3d_kernel_names().decode().split(",")

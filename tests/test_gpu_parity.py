@@ -346,6 +346,34 @@ def test_deposit_shape_variants(PM3D, N, prec, monkeypatch):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_live_shape_trials_keep_bits(PM3D, prec, monkeypatch):
+    """The step deposits try every candidate column shape every 64 steps (PM3D_DEPOSIT_SHAPE=all
+    makes every fitting shape a candidate) and may move to another; the trajectory and every
+    step's density maximum agree bit for bit with a run held to one shape."""
+    from pbh_amd import ic
+    say("live shape trials", prec)
+    spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
+    st = ic.make(spec)
+    outs, trials = {}, {}
+    for v in ("0", "all"):
+        monkeypatch.setenv("PM3D_DEPOSIT_SHAPE", v)
+        pm = PM3D(64, 1.0, 64, precision=prec)
+        pm.set_state(st["x"], st["p"])
+        a = st["a_i"]
+        maxes = []
+        for _ in range(150):
+            a, dm = pm.step(a, 0.004 * a)
+            maxes.append(dm.max())
+        outs[v] = (pm.x, pm.p, np.asarray(dm), np.array(maxes))
+        trials[v] = pm.deposit_trials()
+        pm.close()
+    assert trials["0"]["trials"] == 0
+    assert trials["all"]["trials"] >= 2 and bin(trials["all"]["candidates"]).count("1") >= 2, trials
+    for u, w in zip(outs["0"], outs["all"]):
+        assert np.array_equal(u, w)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_deferred_gather_is_invisible(PM3D, prec):
     """A step leaves its forces as gradient meshes and the next step gathers them in its first
     kernel.  Reading p after every step (gather + kick applied on their own), calling forces()
