@@ -122,12 +122,12 @@ class Run:
     """One box of a workload, resettable to its initial state: `reset()` puts the initial state
     back, `step()` advances with da = da_frac * a."""
 
-    def __init__(self, spec, precision: str, da_frac: float, host_ic: bool):
+    def __init__(self, spec, precision: str, da_frac: float, host_ic: bool, frac_bits: int = 0):
         from pbh_amd import ic
         from pbh_amd.pm3d import PM3D
         self.spec, self.precision, self.da_frac = spec, precision, da_frac
         self.n = spec.n_side ** 3
-        self.pm = PM3D(spec.N, spec.L, spec.n_side, precision=precision)
+        self.pm = PM3D(spec.N, spec.L, spec.n_side, precision=precision, frac_bits=frac_bits)
         if host_ic:
             self.st = ic.make(spec)
             self.eng = None
@@ -259,6 +259,23 @@ def bench_mixed256(K: int, W: int) -> dict:
     return out
 
 
+def bench_fp64_frac40(K: int, W: int) -> dict:
+    """The headline workload with the FP64 deposit on 64-bit cells at 2^-40 (frac_bits = 40, opt-in)."""
+    from pbh_amd import ic
+    spec = ic.named("peak128")
+    run = Run(spec, "fp64", 0.0125, host_ic=True, frac_bits=40)
+    r = late_window(run, K, W, "peak128")
+    n = run.n
+    out = {"workload": "peak128 as the headline, FP64 with the deposit on 64-bit cells at 2^-40 "
+                       "(frac_bits=40: half the mesh bytes, about 1e-12 of a particle per cell)",
+           "crossing_step": r["S"], "ms_per_step": r["late_wall"] / K * 1e3,
+           "particle_steps_per_s": n * K / r["late_wall"],
+           "kernels_avg_us": {k: v["avg_us"] for k, v in r["late_tab"]["kernels"].items()},
+           "early_window": {"ms_per_step": r["early_wall"] / K * 1e3, "particle_steps_per_s": n * K / r["early_wall"]}}
+    run.pm.close()
+    return out
+
+
 def bench_batched() -> dict:
     from pbh_amd.bench_batch import batched_vs_plain
     r = batched_vs_plain()
@@ -362,6 +379,8 @@ def main():
     out["attainable"] = {**att, "note": "hand-written streaming copy / read of 1 GiB (pm3d_stream_probe)"}
     out["roofline"]["frac_of_attainable_copy"] = out["roofline"]["achieved"] / att["copy_gbs"]
     if not args.no_extras:
+        say("FP64 on 64-bit cells")
+        out["fp64_frac40"] = bench_fp64_frac40(K, W)
         say("mixed 256^3")
         out["mixed256"] = bench_mixed256(K, W)
         say("batched 64 peaks")

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define PM3D_FP64 0   /* FP64 positions/momenta/FFT; int64 fixed-point deposit */
+#define PM3D_FP64 0   /* FP64 positions/momenta/FFT; fixed-point deposit (128-bit cells at 2^-52, or 64-bit at 2^-40) */
 #define PM3D_MIXED 1  /* int32 cell + FP32 offset, FP32 momenta/FFT/forces; int64 deposit; FP64 scalars */
 
 #define PM3D_OK 0
@@ -41,8 +41,10 @@ typedef struct pm3d_cfg {
     double L;           /* periodic box side */
     double smooth;      /* Gaussian force smoothing length in cells, 0 = none (pm3d.py:27,107-108) */
     int32_t precision;  /* PM3D_FP64 or PM3D_MIXED */
-    int32_t frac_bits;  /* fixed-point fraction bits of the deposit: 0 (or the mode's own value) selects
-                         * 52 on 128-bit cells (FP64) or 31 on 64-bit cells (mixed); others are rejected */
+    int32_t frac_bits;  /* fixed-point fraction bits of the deposit: 0 = the mode's default (mixed 31 on
+                         * 64-bit cells, FP64 52 on 128-bit cells); FP64 also accepts 40 (64-bit cells,
+                         * Np^3 <= 2^23 so that no cell can overflow: half the mesh bytes, an absolute
+                         * error of about 1e-12 of a particle per cell); others are rejected */
 } pm3d_cfg;
 
 /* Host scalars of one KDK step, evaluated in Python exactly as pm3d.py:118-127 does
@@ -80,7 +82,8 @@ int pm3d_get_state(void* handle, double* x, double* p);  /* either pointer may b
  * deposit's flat index of corner 0, pm3d.py:88) and frac[i,d] = g - floor(g). */
 int pm3d_cic_index(void* handle, const double* X, int64_t n, int64_t* key, double* frac);
 
-/* PM3D.deposit(X) (pm3d.py:81-89): particle counts per cell, (N,N,N) FP64, from X (n,3). */
+/* PM3D.deposit(X) (pm3d.py:81-89): particle counts per cell, (N,N,N) FP64, from X (n,3), on the
+ * handle's fixed-point grid (at most 2^23 particles on 64-bit cells at 2^-40: PM3D_EINVAL beyond). */
 int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho);
 
 /* Allocation-free variant of pm3d_deposit, with no handle: X (n,3) FP64 and rho (N,N,N) FP64 are
@@ -90,7 +93,8 @@ int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho);
  * hipFree or allocating library routine: the fixed-point mesh lives in a device array of the
  * library of 2 x 128^3 uint64 (32 MiB), held by every GPU the library runs on until the process
  * ends, so N is limited to 128 (power of two in [8,128], or 48, 96; otherwise PM3D_EINVAL).  The
- * result is bit for bit that of pm3d_deposit on an FP64 handle (same 2^-52 fixed point).  Every call
+ * result is bit for bit that of pm3d_deposit on an FP64 handle created with frac_bits = 52 (the same
+ * 2^-52 grid on 128-bit cells).  Every call
  * on one GPU shares that array: calls that overlap on one GPU (other streams, other threads) race
  * and must be serialised by the caller.  Errors are read with pm3d_last_error(NULL). */
 int pm3d_deposit_noalloc(const double* X, int64_t n, int32_t N, double L, double* rho, void* stream);
@@ -188,6 +192,9 @@ int pm3d_fft_tiles(void* handle, int32_t* variants, int nmax);
  * Returns 1 for columns, 0 for the atomic path.  No reference counterpart (pm3d.py:81-89 deposits
  * with one np.bincount per CIC corner). */
 int pm3d_deposit_tiles(void* handle, int32_t* shape);
+
+/* Fixed-point fraction bits of the handle's deposit (the frac_bits pm3d_create chose), or < 0. */
+int pm3d_frac_bits(void* handle);
 
 /* Shape trials of the step deposits so far: out[0] = finished trials, out[1] = shape changes they
  * made, out[2] = candidate mask (bit v: shape variant v; fewer than two bits: no trials).  No

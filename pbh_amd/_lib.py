@@ -75,6 +75,7 @@ SIGNATURES = {
     "pm3d_fft_tiles": (c_int, [c_void_p, POINTER(c_int32), c_int]),
     "pm3d_deposit_tiles": (c_int, [c_void_p, POINTER(c_int32)]),
     "pm3d_deposit_trials": (c_int, [c_void_p, POINTER(c_int64)]),
+    "pm3d_frac_bits": (c_int, [c_void_p]),
     "pm3d_device_info": (c_int, [c_void_p, c_char_p, c_int, c_char_p, c_int, PI64]),
     "pm3d_set_analysis_sets": (c_int, [c_void_p, POINTER(c_int32), PI64, c_int]),
     "pm3d_analyse": (c_int, [c_void_p, PD, c_int, PD, PD, PI64, PD]),

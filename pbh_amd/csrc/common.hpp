@@ -291,6 +291,22 @@ __device__ __forceinline__ unsigned long long q52(double w) {
     return (unsigned long long)__double_as_longlong(fma(w, 0x1p52, 0x1p52)) - 0x4330000000000000ull;
 }
 
+// FP64 mode on 64-bit cells: 2^-40 of a particle per corner weight.  A cell holds at most the
+// total n * (2^40 + 4) (each of a particle's 8 rounded weights is at most 1/2 unit high), below 2^64
+// while n <= 2^23; larger FP64 boxes keep the 128-bit cells.
+constexpr int kCompactFB = 40;
+constexpr long long kCompactMaxN = 1LL << 23;
+
+// Round-to-nearest-even of ws as an integer, for ws in [0, 2^52]: ws + 2^52 lands in [2^52, 2^53],
+// where the doubles are the integers (the same rounding as rint).
+__device__ __forceinline__ unsigned long long q_scaled(double ws) {
+    return (unsigned long long)__double_as_longlong(ws + 0x1p52) - 0x4330000000000000ull;
+}
+// The same of w * qs (qs a power of two, w * qs <= 2^52), in one FMA.
+__device__ __forceinline__ unsigned long long q_fixed(double w, double qs) {
+    return (unsigned long long)__double_as_longlong(fma(w, qs, 0x1p52)) - 0x4330000000000000ull;
+}
+
 template <bool WIDE>
 __device__ __forceinline__ void fx_atomic_add(unsigned long long* mesh, size_t cell, unsigned long long q) {
     if constexpr (WIDE) {

@@ -33,7 +33,10 @@ __device__ __forceinline__ T ld_off(const T* base, unsigned byte_offset) {
 // only when it returns true (base cell = the site or the site below it); `axes` is the general CIC
 // of the far pass; `quant*` round one corner weight (wx*wy)*wz (the reference's order,
 // pm3d.py:85-88) to the fixed-point grid of the mesh.
-struct SrcF64 {
+// FB: fraction bits of the mesh, kWideFB (128-bit cells, accumulated as 32-bit halves: quant2) or
+// kCompactFB (64-bit cells: the scale folded into wx, quant_scaled).
+template <int FB>
+struct SrcF64T {
     const double* x0;
     const double* x1;
     const double* x2;
@@ -70,9 +73,15 @@ struct SrcF64 {
             w1[d] = f;
         }
     }
-    // weight scale folded into wx before the products (1 here: q52 scales inside its FMA)
-    static constexpr double kScale = 1.0;
-    __device__ __forceinline__ unsigned long long quant(double w) const { return q52(w); }
+    // weight scale folded into wx before the products (1 for 128-bit cells: q52 scales inside its
+    // FMA).  2^FB (ws = (2^FB wx * wy) * wz) is exactly 2^FB ((wx * wy) * wz): a power-of-two scale
+    // commutes with rounding, and a product that underflows rounds to 0 either way.
+    static constexpr double kScale = FB == kWideFB ? 1.0 : (double)(1ull << FB);
+    __device__ __forceinline__ unsigned long long quant(double w) const {
+        if constexpr (FB == kWideFB) return q52(w);
+        else return q_fixed(w, kScale);
+    }
+    __device__ __forceinline__ unsigned long long quant_scaled(double ws) const { return q_scaled(ws); }
     // q52 as its two 32-bit halves: 0x4330000000000000 has zero low bits, so no borrow crosses
     __device__ __forceinline__ void quant2(double w, unsigned& lo32, unsigned& hi32) const {
         const unsigned long long b = (unsigned long long)__double_as_longlong(fma(w, 0x1p52, 0x1p52));
@@ -80,6 +89,8 @@ struct SrcF64 {
         hi32 = (unsigned)(b >> 32) - 0x43300000u;
     }
 };
+using SrcF64 = SrcF64T<kWideFB>;
+using SrcF64C = SrcF64T<kCompactFB>;
 
 struct SrcMx {
     const int* c0;

@@ -54,9 +54,10 @@ __global__ __launch_bounds__(kThreads) void k_deposit_atomic(Strided X, long lon
 #pragma unroll
     for (int d = 0; d < 3; ++d) cic_axis_fast(X(d, i), dx, N, lo[d], hi[d], f[d]);
     cic_weights(f, w);
-    // the grid of the mesh: 2^-52 on 128-bit cells, 2^-31 on 64-bit cells (the mixed mode's)
+    // the grid of the mesh: 2^-52 on 128-bit cells; 1/qs on 64-bit cells (2^-40 in FP64 mode, 2^-31
+    // in mixed mode)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) q[c] = WIDE ? (long long)q52(w[c]) : (long long)rint(w[c] * 2147483648.0);
+    for (int c = 0; c < 8; ++c) q[c] = WIDE ? (long long)q52(w[c]) : (long long)q_fixed(w[c], qs);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         if (q[c] == 0) continue;

@@ -505,7 +505,10 @@ int fft_launch_variant(Handle* h, int pass, int v) {
 // The choice is cached per (N, precision) for the process.  PM3D_FFT_TILES = one digit 0-3 forces
 // that variant for every pass, five digits one per pass (r2c_z, y fwd, x, y inv, c2r_z).
 std::mutex g_tune_mu;
-std::map<std::pair<int, bool>, std::array<int, FP_COUNT>> g_tuned;
+std::map<std::pair<int, int>, std::array<int, FP_COUNT>> g_tuned;
+// key of the tuning caches: mesh size and kind (0: FP64 on 128-bit cells, 1: mixed, 2: FP64 on
+// 64-bit cells)
+std::pair<int, int> tune_key(const Handle* h) { return {h->N, h->mixed ? 1 : h->wide ? 0 : 2}; }
 
 int tune_fft(Handle* h) {
     const char* env = std::getenv("PM3D_FFT_TILES");
@@ -519,7 +522,7 @@ int tune_fft(Handle* h) {
         }
         return 0;
     }
-    const std::pair<int, bool> key{h->N, h->mixed};
+    const std::pair<int, int> key = tune_key(h);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
@@ -578,6 +581,9 @@ int ensure_sbuf(Handle* h, size_t bytes) {
 // Particle sources of the column deposit: per-axis base pointers of the SoA state.
 SrcF64 src_f64(const Handle* h) {
     return SrcF64{h->x, h->x + h->n, h->x + 2 * h->n, h->dx, 1.0 / h->dx, h->N};
+}
+SrcF64C src_f64c(const Handle* h) {
+    return SrcF64C{h->x, h->x + h->n, h->x + 2 * h->n, h->dx, 1.0 / h->dx, h->N};
 }
 SrcMx src_mx(const Handle* h) {
     return SrcMx{h->cell, h->cell + h->n, h->cell + 2 * h->n, h->off, h->off + h->n, h->off + 2 * h->n, h->N};
@@ -689,11 +695,16 @@ int deposit_into(Handle* h, unsigned long long* mesh, bool lists = false, int v 
     int rc = 0;
     if (!h->mixed) {
         if (tiled) {
-            rc = deposit_columns_n<true>(h, src_f64(h), mesh, v, lists);
+            rc = h->wide ? deposit_columns_n<true>(h, src_f64(h), mesh, v, lists)
+                         : deposit_columns_n<false>(h, src_f64c(h), mesh, v, lists);
         } else {
             Timed t(h, KC_DEPOSIT);
-            k_deposit_atomic<true>
-                <<<grid_for(h->n), kThreads, 0, h->st>>>(Strided{h->x, h->n, 1}, h->n, h->N, h->dx, mesh, h->qs, h->qone);
+            if (h->wide)
+                k_deposit_atomic<true><<<grid_for(h->n), kThreads, 0, h->st>>>(Strided{h->x, h->n, 1}, h->n, h->N, h->dx,
+                                                                               mesh, h->qs, h->qone);
+            else
+                k_deposit_atomic<false><<<grid_for(h->n), kThreads, 0, h->st>>>(Strided{h->x, h->n, 1}, h->n, h->N,
+                                                                                h->dx, mesh, h->qs, h->qone);
         }
     } else {
         if (tiled) {
@@ -780,7 +791,7 @@ struct DepTuned {
     int v;
     unsigned cand;
 };
-std::map<std::pair<int, bool>, DepTuned> g_dep_tuned;
+std::map<std::pair<int, int>, DepTuned> g_dep_tuned;
 
 // Event pairs of the live trials, when there are at least two candidates.
 int live_init(Handle* h, unsigned cand) {
@@ -816,7 +827,7 @@ int tune_deposit(Handle* h) {
         pbx = bx, pby = by, nv = v + 1;
     }
     const unsigned every = (1u << nv) - 1u;
-    const std::pair<int, bool> key{h->N, h->mixed};
+    const std::pair<int, int> key = tune_key(h);
     {
         std::unique_lock<std::mutex> lk(g_tune_mu);
         auto it = g_dep_tuned.find(key);
@@ -837,8 +848,9 @@ int tune_deposit(Handle* h) {
         return fail(h, PM3D_EHIP, "hipEventCreate failed");
     }
     auto launch = [&](int v) {
-        return h->mixed ? deposit_columns_n<false>(h, src_mx(h), h->mesh, v, false)
-                        : deposit_columns_n<true>(h, src_f64(h), h->mesh, v, false);
+        return h->mixed  ? deposit_columns_n<false>(h, src_mx(h), h->mesh, v, false)
+               : h->wide ? deposit_columns_n<true>(h, src_f64(h), h->mesh, v, false)
+                         : deposit_columns_n<false>(h, src_f64c(h), h->mesh, v, false);
     };
     // three interleaved rounds of 4 launches per shape (after one warm-up each); a shape's time is
     // its fastest round, and a narrower shape replaces the widest only when 3 % faster
@@ -1341,17 +1353,28 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         return PM3D_EINVAL;
     }
     const bool mixed = cfg->precision == PM3D_MIXED;
-    // FP64 mode: 128-bit cells at 2^-52.  Mixed mode: 64-bit cells at 2^-31 (cannot overflow).
-    const int fb = mixed ? kNarrowFB : kWideFB;
-    if (cfg->frac_bits != 0 && cfg->frac_bits != fb) {
-        g_create_error = "frac_bits is fixed by the precision mode (FP64: 52 on 128-bit cells, mixed: 31); pass 0";
-        return PM3D_EINVAL;
+    // Mixed mode: 64-bit cells at 2^-31 (cannot overflow).  FP64 mode: 128-bit cells at 2^-52, or on
+    // request (frac_bits = 40, n <= 2^23 so that no cell can overflow) 64-bit cells at 2^-40, whose
+    // absolute error (about 1e-12 of a particle per cell) is far above FP64 rounding against the
+    // density contrast of a nearly uniform state.
+    const long long np3 = (long long)cfg->Np * cfg->Np * cfg->Np;
+    const bool compact_ok = np3 <= kCompactMaxN;
+    int fb = mixed ? kNarrowFB : kWideFB;
+    if (cfg->frac_bits != 0) {
+        const bool ok = mixed ? cfg->frac_bits == kNarrowFB
+                              : cfg->frac_bits == kWideFB || (cfg->frac_bits == kCompactFB && compact_ok);
+        if (!ok) {
+            g_create_error = "frac_bits: mixed 31; FP64 52 (128-bit cells) or 40 (64-bit cells, at most 2^23 "
+                             "particles); 0 = the mode's default (31, 52)";
+            return PM3D_EINVAL;
+        }
+        fb = cfg->frac_bits;
     }
     Handle* h = new Handle();
     h->cfg = *cfg;
     h->cfg.frac_bits = fb;
     h->mixed = mixed;
-    h->wide = !mixed;
+    h->wide = fb == kWideFB;
     h->N = N;
     h->log2N = lg;
     h->Np = cfg->Np;
@@ -1563,6 +1586,8 @@ int pm3d_cic_index(void* handle, const double* X, int64_t n, int64_t* key, doubl
 int pm3d_deposit(void* handle, const double* X, int64_t n, double* rho) {
     if (!handle || !rho || (n > 0 && !X) || n < 0) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
+    if (!h->mixed && !h->wide && n > kCompactMaxN)
+        return fail(h, PM3D_EINVAL, "more than 2^23 particles on 64-bit cells at 2^-40 (frac_bits = 52 gives 128-bit cells)");
     const size_t xb = 3 * (size_t)n * sizeof(double);
     int rc = ensure_sbuf(h, std::max(xb, (size_t)h->M * sizeof(double)));
     if (rc) return rc;
@@ -2023,6 +2048,8 @@ int pm3d_deposit_tiles(void* handle, int32_t* shape) {
     shape[2] = h->N;
     return 1;
 }
+
+int pm3d_frac_bits(void* handle) { return handle ? ((Handle*)handle)->cfg.frac_bits : PM3D_EINVAL; }
 
 int pm3d_deposit_trials(void* handle, int64_t* out) {
     if (!handle || !out) return PM3D_EINVAL;

@@ -124,6 +124,7 @@ class PM3D:
         self._acc_a = None
         self._kgrid = None
         self.n = Np ** 3
+        self.frac_bits = int(self._L.pm3d_frac_bits(h))  # the deposit's grid is 2^-frac_bits of a particle
 
     def close(self):
         """Release the handle's device memory now (otherwise when the object is collected)."""

@@ -47,3 +47,8 @@ def test_create_error_path_without_gpu():
     assert b"power of two" in lib.pm3d_last_error(None)
     cfg = _lib.Pm3dCfg(N=16, Np=16, L=-1.0, smooth=0.0, precision=0, frac_bits=0)
     assert lib.pm3d_create(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    # fraction bits: FP64 accepts 52 and 40 (40 only up to 2^23 particles), mixed only 31
+    for prec, fb, np_side in ((0, 40, 256), (0, 31, 16), (1, 40, 16), (1, 52, 16)):
+        cfg = _lib.Pm3dCfg(N=16, Np=np_side, L=1.0, smooth=0.0, precision=prec, frac_bits=fb)
+        assert lib.pm3d_create(ctypes.byref(cfg), ctypes.byref(h)) == -1, (prec, fb, np_side)
+        assert b"frac_bits" in lib.pm3d_last_error(None)

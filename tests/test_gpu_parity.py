@@ -67,11 +67,13 @@ def test_cic_keys_bit_exact(PM3D, N, L):
 
 
 # ------------------------------------------------------------------------------- deposit
-@pytest.mark.parametrize("N,n", [(8, 1), (16, 5000), (64, 300000), (128, 2097152)])
-def test_deposit_parity(PM3D, N, n):
-    say("deposit", N, n)
+@pytest.mark.parametrize("N,n,fb", [(8, 1, 0), (16, 5000, 0), (64, 300000, 0), (128, 2097152, 0),
+                                    (16, 5000, 40), (128, 2097152, 40)])
+def test_deposit_parity(PM3D, N, n, fb):
+    """fb 0: 128-bit cells at 2^-52; fb 40: 64-bit cells at 2^-40 (about 1e-12 of a particle)."""
+    say("deposit", N, n, fb)
     L = 1.0
-    pm = PM3D(N, L, 8)
+    pm = PM3D(N, L, 8, frac_bits=fb)
     X = np.random.default_rng(n).uniform(0, L, (n, 3))
     rho = pm.deposit(X)
     ref = O.deposit(X, L / N, N)
@@ -189,11 +191,11 @@ def test_analyse_matches_golden(PM3D, golden):
 
 
 # ------------------------------------------------------------ trajectories on the workloads
-def _trajectory_case(PM3D, spec, nsteps, da_frac):
+def _trajectory_case(PM3D, spec, nsteps, da_frac, frac_bits=0):
     from pbh_amd import ic
     st = ic.make(spec)
     L = spec.L
-    pm = PM3D(spec.N, L, spec.n_side)
+    pm = PM3D(spec.N, L, spec.n_side, frac_bits=frac_bits)
     pm.set_state(st["x"], st["p"])
     ref = O.OraclePM3D(spec.N, L, spec.n_side)
     ref.x, ref.p = st["x"].copy(), st["p"].copy()
@@ -213,19 +215,21 @@ def _trajectory_case(PM3D, spec, nsteps, da_frac):
     return pm, ref
 
 
-def test_trajectory_peak32_parity(PM3D):
+@pytest.mark.parametrize("fb", [0, 40])
+def test_trajectory_peak32_parity(PM3D, fb):
     from pbh_amd import ic
-    say("trajectory peak32")
+    say("trajectory peak32", fb)
     spec = ic.PeakSpec(N=32, mu=0.3, sigma=(0.12, 0.11, 0.10), seed=0)
-    pm, ref = _trajectory_case(PM3D, spec, 40, 0.02)
+    pm, ref = _trajectory_case(PM3D, spec, 40, 0.02, fb)
     assert pm.min_jacobian() == pytest.approx(O.min_jacobian(ref.x, 32, 1.0), rel=1e-9)
 
 
-def test_trajectory_peak128_parity(PM3D):
+@pytest.mark.parametrize("fb", [0, 40])
+def test_trajectory_peak128_parity(PM3D, fb):
     """128^3 mesh, 2.1M particles, FP64 on the GPU vs the CPU oracle (3 steps from the IC)."""
     from pbh_amd import ic
-    say("trajectory peak128")
-    _trajectory_case(PM3D, ic.named("peak128"), 3, 0.0125)
+    say("trajectory peak128", fb)
+    _trajectory_case(PM3D, ic.named("peak128"), 3, 0.0125, fb)
 
 
 def test_step_is_deterministic(PM3D):
@@ -244,13 +248,14 @@ def test_step_is_deterministic(PM3D):
         assert np.array_equal(u, v)
 
 
-def test_tiled_deposit_equals_atomic(PM3D):
+@pytest.mark.parametrize("fb", [0, 40])
+def test_tiled_deposit_equals_atomic(PM3D, fb):
     """The step's owner-computes deposit and the reference-shaped atomic deposit give the same mesh."""
     from pbh_amd import ic
-    say("tiled vs atomic")
+    say("tiled vs atomic", fb)
     spec = ic.PeakSpec(N=64, mu=0.5, sigma=(0.08, 0.07, 0.06))
     st = ic.make(spec)
-    pm = PM3D(64, 1.0, 64)
+    pm = PM3D(64, 1.0, 64, frac_bits=fb)
     pm.set_state(st["x"], st["p"])
     a = 0.01
     for k in range(60):          # run well into the non-linear regime so that 'far' particles exist
@@ -444,18 +449,19 @@ def test_kernel_timing_is_invisible_and_consistent(PM3D, prec):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_global_sums(PM3D, prec):
+@pytest.mark.parametrize("prec,fb", [("fp64", 0), ("fp64", 40), ("mixed", 0)])
+def test_global_sums(PM3D, prec, fb):
     """pm3d_totals: the deposit's mass is n plus the exact deviation of the fixed-point total
-    (FP64 mode: recomputed on the host as the integer sum over particles of sum_c rint(w_c 2^52) -
-    2^52 from the oracle's corner weights, which equal the kernel's on identical positions; the
-    weights of a particle need not sum to exactly 1 in FP64), and the total momentum matches the
-    oracle's sum of p; the same state gives the same bits twice."""
+    (FP64 mode: recomputed on the host as the integer sum over particles of sum_c rint(w_c 2^fb) -
+    2^fb from the oracle's corner weights, which equal the kernel's on identical positions, fb the
+    handle's fraction bits (40 on 64-bit cells, 52 on 128-bit); the weights of a particle need not sum
+    to exactly 1 in FP64), and the total momentum matches the oracle's sum of p; the same state gives
+    the same bits twice."""
     from pbh_amd import ic
-    say("global sums", prec)
+    say("global sums", prec, fb)
     spec = ic.PeakSpec(N=64, mu=0.3, sigma=(0.12, 0.11, 0.10))
     st = ic.make(spec)
-    pm = PM3D(64, 1.0, 64, precision=prec)
+    pm = PM3D(64, 1.0, 64, precision=prec, frac_bits=fb)
     pm.set_state(st["x"], st["p"])
     ref = O.OraclePM3D(64, 1.0, 64)
     ref.x, ref.p = st["x"].copy(), st["p"].copy()
@@ -472,12 +478,13 @@ def test_global_sums(PM3D, prec):
     dmom = np.abs(t1["momentum"] - ref.p.sum(axis=0)) / scale
     say(f"{prec}: mass - n = {t1['mass_dev']:.3e}, momentum {t1['momentum']}, rel diff {dmom.max():.2e}")
     if prec == "fp64":
-        x = pm.x
-        units = np.zeros(len(x), np.int64)  # per particle sum_c q_c <= 8 * 2^52: exact in int64
+        x, bits = pm.x, pm.frac_bits
+        assert bits == (fb or 52)
+        units = np.zeros(len(x), np.int64)  # per particle sum_c q_c <= 8 * 2^bits: exact in int64
         for ix, iy, iz, wx, wy, wz in O._corners(x, 1.0 / 64, 64):
-            units += np.rint((wx * wy * wz) * 2.0 ** 52).astype(np.int64)
-        dev_units = int((units - (1 << 52)).sum())
-        assert t1["mass_dev"] == dev_units * 2.0 ** -52
+            units += np.rint((wx * wy * wz) * 2.0 ** bits).astype(np.int64)
+        dev_units = int((units - (1 << bits)).sum())
+        assert t1["mass_dev"] == dev_units * 2.0 ** -bits
         assert dmom.max() <= 1e-12
     else:
         assert abs(t1["mass_dev"]) <= 8 * n * 2.0 ** -32

@@ -74,7 +74,7 @@ def test_noalloc_deposit_matches_handle_deposit(hip):
         dR.get(got)
         dX.free()
         dR.free()
-        ref = PM3D(N, L, 8).deposit(X)
+        ref = PM3D(N, L, 8, frac_bits=52).deposit(X)  # the same 2^-52 grid on 128-bit cells
         assert np.array_equal(got, ref)
         assert abs(got.sum() - n) <= 1e-9 * n
 
