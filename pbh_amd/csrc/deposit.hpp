@@ -417,6 +417,14 @@ __device__ __forceinline__ void deposit_far_bal_body(Src src, unsigned long long
                                                      const int* __restrict__ far,
                                                      const unsigned int* __restrict__ farcnt, int ncol, int seg,
                                                      int N, unsigned blk, unsigned nblk, FarRows fr, int lists) {
+    if (lists && fr.cnt) {
+        // the columns took the far particles from the row lists (and recorded none themselves): only
+        // the lists' overflow and the clearing of the row counts are left
+        const unsigned nov = min(*fr.ovcnt, fr.ovcap);
+        for (unsigned t = blk * kThreads + threadIdx.x; t < nov; t += nblk * kThreads) far_add<Src, WIDE>(src, mesh, fr.ovl[t], N);
+        for (unsigned r = blk * kThreads + threadIdx.x; r < (unsigned)N * (unsigned)N; r += nblk * kThreads) fr.cnt[r] = 0u;
+        return;
+    }
     __shared__ unsigned int pre[kFarColsMax + 1];
     __shared__ unsigned int wsum[kThreads / 64];
     const int R = (ncol + kThreads - 1) / kThreads;  // columns per thread, contiguous
@@ -453,11 +461,8 @@ __device__ __forceinline__ void deposit_far_bal_body(Src src, unsigned long long
         }
         far_add<Src, WIDE>(src, mesh, far[(size_t)a * seg + (t - pre[a])], N);
     }
-    if (fr.cnt) {
-        const unsigned nov = lists ? min(*fr.ovcnt, fr.ovcap) : 0u;
-        for (unsigned t = blk * kThreads + threadIdx.x; t < nov; t += nblk * kThreads) far_add<Src, WIDE>(src, mesh, fr.ovl[t], N);
+    if (fr.cnt)  // row lists that this deposit did not take: cleared all the same
         for (unsigned r = blk * kThreads + threadIdx.x; r < (unsigned)N * (unsigned)N; r += nblk * kThreads) fr.cnt[r] = 0u;
-    }
 }
 
 template <class Src, bool WIDE>

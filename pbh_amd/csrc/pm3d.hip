@@ -594,6 +594,11 @@ SrcMx src_mx(const Handle* h) {
 unsigned far_bal_grid(long long n) {
     return (unsigned)std::min<long long>(kFarBalGrid, (n + kThreads - 1) / kThreads);
 }
+// With row lists the pass only deposits their overflow (rare) and clears the N^2 row counts: one
+// count per thread.
+unsigned far_lists_grid(int N) {
+    return (unsigned)std::max<long long>(1, std::min<long long>(kFarBalGrid, (long long)N * N / kThreads));
+}
 
 // Column deposit (deposit.hpp) for one mesh size and column shape variant; `lists`: take the far
 // particles from the row lists the step's gather-kick-drift built.
@@ -608,8 +613,8 @@ void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh, bool l
     }
     Timed t(h, KC_DEPOSIT_FAR, true);
     if constexpr (S::NBLK <= kFarColsMax) {
-        t.launch(k_deposit_far_bal<Src, WIDE>, far_bal_grid(h->n), kThreads, src, mesh, h->far, h->farcnt, S::NBLK,
-                 S::BX * S::BY * N, N, fr, lists ? 1 : 0);
+        t.launch(k_deposit_far_bal<Src, WIDE>, lists ? far_lists_grid(N) : far_bal_grid(h->n), kThreads, src, mesh,
+                 h->far, h->farcnt, S::NBLK, S::BX * S::BY * N, N, fr, lists ? 1 : 0);
     } else {
         // a column lists at most its own BX*BY*N sites: kFarSub workgroups per column bound the work
         // of the fullest one
@@ -1172,7 +1177,8 @@ void launch_boxes_deposit(Handle* h0, const MxBox* dev, int nb, hipStream_t st) 
     using S = ColShape<N, false, V>;
     k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, XMap{1, 1});
     if constexpr (S::NBLK <= kFarColsMax)
-        k_deposit_far_bal_boxes<<<dim3(far_bal_grid(h0->n), nb), kThreads, 0, st>>>(dev, S::NBLK, S::BX * S::BY * N);
+        k_deposit_far_bal_boxes<<<dim3(h0->rowcnt ? far_lists_grid(N) : far_bal_grid(h0->n), nb), kThreads, 0, st>>>(
+            dev, S::NBLK, S::BX * S::BY * N);
     // (the grouped gather-kick-drift of the same replay built every member's row lists)
     else
         k_deposit_far_seg_boxes<<<dim3(S::NBLK, kFarSub, nb), kThreads, 0, st>>>(dev, S::BX * S::BY * N);
@@ -1433,14 +1439,18 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         h->err = "initial copies failed";
         return bail(PM3D_EHIP);
     }
-    // far-particle row lists: column deposit with the balanced far pass (N <= 256); capacity 2N per
-    // row.  PM3D_ROW_CAP = 1 .. 2N overrides it (to exercise the overflow path), 0 turns the lists
-    // off (every far particle through the global-atomic pass); all give the same mesh bits.
-    int cap = (columns_used(h) && N <= 256) ? 2 * N : 0;
+    // far-particle row lists: column deposit with the balanced far pass (N <= 256); capacity per row
+    // = what the acceleration buffer they alias holds, 6N (FP64: 24 B per particle) or 3N (mixed: 12
+    // B) int entries for Np = N.  PM3D_ROW_CAP = 1 .. that overrides it (to exercise the overflow
+    // path), 0 turns the lists off (every far particle through the global-atomic pass); all give the
+    // same mesh bits.
+    const int cap_max = (int)(3 * (size_t)h->n * (mixed ? sizeof(float) : sizeof(double)) / sizeof(int) /
+                              ((size_t)N * N));
+    int cap = (columns_used(h) && N <= 256) ? cap_max : 0;
     if (const char* env = std::getenv("PM3D_ROW_CAP")) {
         const int c = std::atoi(env);
-        if (c < 0 || c > 2 * N) {
-            h->err = "PM3D_ROW_CAP must be in [0, 2N]";
+        if (c < 0 || c > cap_max) {
+            h->err = "PM3D_ROW_CAP must be in [0, 6N] (FP64) or [0, 3N] (mixed)";
             return bail(PM3D_EINVAL);
         }
         if (cap) cap = c;
