@@ -897,13 +897,16 @@ double inv_c3(const Handle* h) {
     return 1.0 / (c * c * c);
 }
 
-// Gather blocks of kThreads consecutive particles: one x plane of the Lagrangian lattice is
-// Np^2/kThreads blocks; an XCD keeps 1/8 of every plane, so its corner rows stay in its L2.
+// Gather blocks of kThreads consecutive particles, dealt to the XCDs in runs of 4 blocks (1024
+// particles: a few rows of a Lagrangian x plane, whose corner rows then share an XCD's L2).  Longer
+// runs hold up the collapse: with an eighth of a plane per run (32 blocks at 256^3) the waves of the
+// collapsed core queue on its few gradient lines (L2 -> L1 return stalls 60x the early steps').
+// Measured, mixed 256^3 / FP64 128^3 gather-kick at the crossing (us): run of 1 block 284 / 56.3,
+// 2: 285 / 51.8, 4: 280 / 49.3, 8: 284 / 51.0, 16: 293 / 51.7, 32: 340 / -; early steps 228-234 /
+// 47-50 for runs of 2-8.
 XMap gather_map(Handle* h) {
-    const long long per_plane = (long long)h->Np * h->Np / kThreads;
-    const unsigned share = per_plane >= (long long)kXcds && (long long)h->Np * h->Np % kThreads == 0
-                               ? (unsigned)(per_plane / kXcds) : 1u;
-    return xmap(grid_for(h->n), share);
+    const unsigned G = grid_for(h->n);
+    return xmap(G, G % (kXcds * 4) == 0 ? 4u : 1u);
 }
 
 // Force meshes at the current positions: deposit, Poisson, the three gradient meshes in h->grad.
