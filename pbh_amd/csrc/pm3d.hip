@@ -600,6 +600,17 @@ unsigned far_lists_grid(int N) {
     return (unsigned)std::max<long long>(1, std::min<long long>(kFarBalGrid, (long long)N * N / kThreads));
 }
 
+// XCD mapping of the column deposit: one x row of columns (N / BY of them) after another dealt
+// round-robin to the XCDs.  Neighbouring columns of a row share ring lines in one L2, and the heavy
+// columns of a collapsing core are spread over all XCDs instead of landing in one XCD's range.
+// Measured at the first shell crossing, mixed 256^3 (8x8 columns) / FP64 128^3 (4x4), us: one
+// contiguous range per XCD 271 / 42, runs of 1-8 columns 249-256 / 37, 16: 228 / 35.6, 32 (a row):
+// 214 / 35.5, 64: 218 / 37.5; early steps within 1 us of each other (151 / 27).
+template <int N, int BY>
+XMap column_map(unsigned ncol) {
+    return xmap(ncol, (unsigned)(N / BY));
+}
+
 // Column deposit (deposit.hpp) for one mesh size and column shape variant; `lists`: take the far
 // particles from the row lists the step's gather-kick-drift built.
 template <int N, bool WIDE, int V, class Src>
@@ -609,7 +620,7 @@ void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh, bool l
     {
         Timed t(h, KC_DEPOSIT, true);
         t.launch(k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024>, S::NBLK, 1024, src, mesh, h->far, h->farcnt,
-                 lists ? fr : FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u}, XMap{1, 1});
+                 lists ? fr : FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u}, column_map<N, S::BY>(S::NBLK));
     }
     Timed t(h, KC_DEPOSIT_FAR, true);
     if constexpr (S::NBLK <= kFarColsMax) {
@@ -1178,7 +1189,7 @@ MxBox box_of(Handle* h) {
 template <int N, int V>
 void launch_boxes_deposit(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
     using S = ColShape<N, false, V>;
-    k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, XMap{1, 1});
+    k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, column_map<N, S::BY>(S::NBLK));
     if constexpr (S::NBLK <= kFarColsMax)
         k_deposit_far_bal_boxes<<<dim3(h0->rowcnt ? far_lists_grid(N) : far_bal_grid(h0->n), nb), kThreads, 0, st>>>(
             dev, S::NBLK, S::BX * S::BY * N);
