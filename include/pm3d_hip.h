@@ -37,7 +37,7 @@ extern "C" {
 
 typedef struct pm3d_cfg {
     int32_t N;          /* mesh cells per side: power of two in [8, 512], or 48 or 96 */
-    int32_t Np;         /* particles per side (n = Np^3) */
+    int32_t Np;         /* particles per side (n = Np^3), 1 .. 812 */
     double L;           /* periodic box side */
     double smooth;      /* Gaussian force smoothing length in cells, 0 = none (pm3d.py:27,107-108) */
     int32_t precision;  /* PM3D_FP64 or PM3D_MIXED */

@@ -41,7 +41,45 @@ struct MxBox {
     double kval;                 // 1/(N dx)
     double inv_c3;               // (Np / N)^3: 1 / (cells per lattice site)^3
     double qscale, nmean;
+    SiteDiv sdv;                 // lattice site of a particle index (row-list test)
 };
+
+// Pointers read from a descriptor in memory are generic to the compiler, which then reaches memory
+// through them with flat instructions: a 64-bit address per lane, no scalar base, and completion
+// counted against the LDS counter as well, so every LDS wait of a kernel (the deposit's atomics and
+// barriers) also waits for its outstanding flat loads.  They all point to global memory; a round
+// trip through the global address space lets the compiler use global instructions.
+template <class T>
+__device__ __forceinline__ T* as_global(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)p;
+}
+
+// Box k's descriptor with every pointer marked global (as_global).
+__device__ __forceinline__ MxBox load_box(const MxBox* __restrict__ boxes, unsigned k) {
+    MxBox b = boxes[k];
+    b.cell = as_global(b.cell);
+    b.off = as_global(b.off);
+    b.p = as_global(b.p);
+    b.acc = as_global(b.acc);
+    b.dscal = as_global(b.dscal);
+    b.sched = as_global(b.sched);
+    b.tab = as_global(b.tab);
+    b.jhist = as_global(b.jhist);
+    b.mesh = as_global(b.mesh);
+    b.far = as_global(b.far);
+    b.farcnt = as_global(b.farcnt);
+    b.spec = as_global(b.spec);
+    b.gspec = as_global(b.gspec);
+    b.grad = as_global(b.grad);
+    b.tw = as_global(b.tw);
+    b.bmax = as_global(b.bmax);
+    b.qslot = as_global(b.qslot);
+    b.fr.cnt = as_global(b.fr.cnt);
+    b.fr.list = as_global(b.fr.list);
+    b.fr.ovl = as_global(b.fr.ovl);
+    b.fr.ovcnt = as_global(b.fr.ovcnt);
+    return b;
+}
 
 // A member whose schedule has ended (table done, or ended at its first shell crossing) skips the
 // step: every kernel of a replay sees the same counter (it changes only in the step's last kernel).
@@ -53,27 +91,27 @@ __device__ __forceinline__ SrcMx box_src(const MxBox& b) {
 
 // The previous step's gather fused with this step's kicks and drift (gather_kick_drift_mx_body).
 __global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx_boxes(const MxBox* __restrict__ boxes, XMap xm) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
     gather_kick_drift_mx_body(b.cell, b.off, b.p, b.n, b.N, b.grad, 1, 0.f, 0.f, 0.f, b.dscal + 1, b.sched, b.tab, xm,
-                              b.fr);
+                              b.fr, b.sdv);
 }
 
 template <int BX, int BY, int N>
 __global__ __launch_bounds__(1024) void k_deposit_col_boxes(const MxBox* __restrict__ boxes, XMap xm) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
     deposit_col_body<SrcMx, BX, BY, N, false, 1024>(box_src(b), b.mesh, b.far, b.farcnt, b.fr, xm);
 }
 
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox* __restrict__ boxes, int seg) {
-    const MxBox& b = boxes[blockIdx.z];
+    const MxBox b = load_box(boxes, blockIdx.z);
     if (box_done(b)) return;
     deposit_far_seg_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, seg, b.N);
 }
 
 __global__ __launch_bounds__(kThreads) void k_deposit_far_bal_boxes(const MxBox* __restrict__ boxes, int ncol, int seg) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
     deposit_far_bal_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, ncol, seg, b.N, blockIdx.x, gridDim.x, b.fr,
                                        1);
@@ -81,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit_far_bal_boxes(const MxBox*
 
 template <int N>
 __global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2);
     fwd_plane_body<float, N, true, false>(b.mesh, b.spec, b.spec + specN, b.tw, b.bmax, b.qscale, b.nmean, xm,
@@ -92,7 +130,7 @@ __global__ __launch_bounds__((PlaneTile<float, N>::NT)) void k_fwd_plane_boxes(c
 // plane forward pass's 2N per-block deposit maxima into the box's slot.
 template <int N, int V>
 __global__ __launch_bounds__(kThreads) void k_x_poisson_boxes(const MxBox* __restrict__ boxes, XMap xm) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
     x_pass_body<float, N, 2, V>(b.spec, b.spec + specN, b.gspec, b.gspec + specN, tot, tot, b.tw, b.kval, 0.0, 0.0, 0,
@@ -101,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void k_x_poisson_boxes(const MxBox* __res
 
 template <int N, int Q>
 __global__ __launch_bounds__((InvTile<float, N, Q>::NT)) void k_inv_plane_boxes(const MxBox* __restrict__ boxes, XMap xm) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
     constexpr size_t specN = (size_t)N * N * (N / 2), tot = specN + (size_t)N * N;
     inv_plane_body<float, N, Q>(b.gspec, b.gspec + specN, tot, tot, b.grad, (size_t)N * N * N, b.tw,
@@ -109,13 +147,13 @@ __global__ __launch_bounds__((InvTile<float, N, Q>::NT)) void k_inv_plane_boxes(
 }
 
 __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx_boxes(const MxBox* __restrict__ boxes) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
-    min_jacobian_mx_body(MxPos{b.cell, b.off, b.n}, b.Np, b.N, b.inv_c3, b.bmax);
+    min_jacobian_mx4_body(MxPos{b.cell, b.off, b.n}, b.N, b.inv_c3, b.sdv, b.bmax);  // grouped members: Np = N
 }
 
 __global__ __launch_bounds__(kThreads) void k_sched_advance_boxes(const MxBox* __restrict__ boxes, int nb) {
-    const MxBox& b = boxes[blockIdx.y];
+    const MxBox b = load_box(boxes, blockIdx.y);
     sched_advance_body(b.sched, b.bmax, nb, b.jhist);
 }
 

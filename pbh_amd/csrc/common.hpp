@@ -16,8 +16,72 @@ struct DevStep {
     double da, den_d;   // drift: x += (da*p)/den_d
     double den2;        // closing kick of this step (applied with the next opening kick)
     double s, sm2;      // Green's function factor -1.5*H_I^2/a_new and (smooth*dx)^2
+    double rdenp, rden1, rden_d;  // 1/denp, 1/den1, 1/den_d, correctly rounded (qdiv)
     float kp, k1, sd, pad;  // mixed mode: c1p/denp, c1/den1, da/(den_d*dx)
 };
+
+// Load / store of base[byte_offset / sizeof(T)] with a 32-bit byte offset from a wave-uniform base
+// (arrays under 4 GiB): one instruction with the base in scalar registers and the offset in one
+// vector register, instead of a 64-bit address computed per lane and per array.
+template <class T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned byte_offset) {
+    return *(const T*)((const char*)base + byte_offset);
+}
+template <class T>
+__device__ __forceinline__ void st_off(T* base, unsigned byte_offset, T v) {
+    *(T*)((char*)base + byte_offset) = v;
+}
+
+// x / y for a divisor y fixed over the launch, with r = 1/y correctly rounded on the host: q0 = x*r
+// is within about an ulp of x/y, and one correction with the exact FMA residual x - q0*y (Markstein,
+// IBM J. Res. Dev. 34 (1990) 111) gives the IEEE quotient outside the under/overflow range (checked
+// bit for bit against x/y on the CPU for the divisors of the KDK schedule, tests/test_div_exact.py).
+// A zero x keeps its sign (-0/y = -0), as the division does.
+__device__ __forceinline__ double qdiv(double x, double y, double r) {
+    const double q0 = x * r;
+    const double q = fma(fma(-q0, y, x), r, q0);
+    return x == 0.0 ? q0 : q;
+}
+
+// Unsigned division by a divisor fixed for the launch (Granlund & Montgomery, PLDI 1994, the
+// round-up method): q = (umulhi(n, m) + n) >> s with m = floor(2^32 (2^s - d) / d) + 1,
+// s = ceil(log2 d); exact for n < 2^31 (umulhi(n, m) <= n, so the sum does not wrap).
+struct Div32 {
+    unsigned m, s;
+};
+inline Div32 div32_make(unsigned d) {
+    unsigned s = 0;
+    while ((1ull << s) < d) ++s;
+    return Div32{(unsigned)(((1ull << 32) * ((1ull << s) - d)) / d + 1), s};
+}
+__device__ __forceinline__ unsigned div32(unsigned n, Div32 v) { return (__umulhi(n, v.m) + n) >> v.s; }
+
+// Lattice site (a, b, c) of particle i = (a*Np + b)*Np + c, with dNp = div32_make(Np) and
+// dNp2 = div32_make(Np*Np).
+struct SiteDiv {
+    Div32 dNp, dNp2;
+    unsigned Np;
+};
+inline SiteDiv site_div(int Np) { return SiteDiv{div32_make((unsigned)Np), div32_make((unsigned)Np * Np), (unsigned)Np}; }
+__device__ __forceinline__ void site_of(unsigned i, const SiteDiv& sd, int s[3]) {
+    const unsigned a = div32(i, sd.dNp2), r = i - a * sd.Np * sd.Np, b = div32(r, sd.dNp);
+    s[0] = (int)a;
+    s[1] = (int)b;
+    s[2] = (int)(r - b * sd.Np);
+}
+
+// c in (-2^30, 2^30) wrapped into [0, N): one compare for a step of less than a box, the remainder
+// otherwise.
+__device__ __forceinline__ int wrap_cell(int c, int N) {
+    if ((unsigned)c >= (unsigned)N) {
+        c = c < 0 ? c + N : c - N;
+        if ((unsigned)c >= (unsigned)N) {
+            c %= N;
+            if (c < 0) c += N;
+        }
+    }
+    return c;
+}
 struct DevSched {
     int counter;  // next entry; advanced by the last kernel of each step
     int nsteps;   // entries in the table

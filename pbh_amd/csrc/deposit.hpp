@@ -22,12 +22,6 @@
 
 namespace pm3d {
 
-// Load of base[byte_offset / sizeof(T)] with a 32-bit byte offset (arrays under 4 GiB).
-template <class T>
-__device__ __forceinline__ T ld_off(const T* base, unsigned byte_offset) {
-    return *(const T*)((const char*)base + byte_offset);
-}
-
 // Particle sources.  `load` issues the raw loads of site i (32-bit index off per-axis base pointers,
 // so every load is one instruction); `near_axis` is the column deposit's fast CIC of one axis, valid
 // only when it returns true (base cell = the site or the site below it); `axes` is the general CIC

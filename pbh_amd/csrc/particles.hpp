@@ -77,37 +77,35 @@ __device__ __forceinline__ void gather3_at(const double X[3], int N, double dx, 
     for (int d = 0; d < 3; ++d) cic_axis_fast(X[d], dx, N, lo[d], hi[d], f[d]);
     const double wx[2] = {1.0 - f[0], f[0]}, wy[2] = {1.0 - f[1], f[1]}, wz[2] = {1.0 - f[2], f[2]};
     const size_t M = (size_t)N * N * N;
+    const double *g0 = grad, *g1 = grad + M, *g2 = grad + 2 * M;  // one component mesh < 4 GiB
+    const unsigned uN = (unsigned)N;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int ix = (c >> 2) ? hi[0] : lo[0], iy = ((c >> 1) & 1) ? hi[1] : lo[1], iz = (c & 1) ? hi[2] : lo[2];
-        const double* g = grad + (((size_t)ix * N + iy) * N + iz);
+        const unsigned o = (((unsigned)ix * uN + (unsigned)iy) * uN + (unsigned)iz) * 8u;
         const double u = wx[c >> 2], v = wy[(c >> 1) & 1], w = wz[c & 1];
-        a0 = a0 + ((g[0] * u) * v) * w;
-        a1 = a1 + ((g[M] * u) * v) * w;
-        a2 = a2 + ((g[2 * M] * u) * v) * w;
+        a0 = a0 + ((ld_off(g0, o) * u) * v) * w;
+        a1 = a1 + ((ld_off(g1, o) * u) * v) * w;
+        a2 = a2 + ((ld_off(g2, o) * u) * v) * w;
     }
     a[0] = a0;
     a[1] = a1;
     a[2] = a2;
 }
 
-// Gather of every particle into acc (SoA).  KICK: p += (c1*acc)/den (pm3d.py:127) as well.
-template <bool KICK>
+// Gather of every particle into acc (SoA).
 __global__ __launch_bounds__(kThreads) void k_gather3(const double* __restrict__ x, long long n, int N, double dx,
                                                       const double* __restrict__ grad, double* __restrict__ acc,
-                                                      double* __restrict__ p, double c1, double den, XMap xm) {
+                                                      XMap xm) {
     const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
     if (i >= n) return;
-    const double X[3] = {x[i], x[n + i], x[2 * n + i]};
+    const unsigned o = (unsigned)i * 8u;  // 3n doubles < 4 GiB
+    const double X[3] = {ld_off(x, o), ld_off(x + n, o), ld_off(x + 2 * n, o)};
     double a[3];
     gather3_at(X, N, dx, grad, a);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d * n + i] = a[d];
-    if (KICK) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) p[d * n + i] = p[d * n + i] + (c1 * a[d]) / den;
-    }
+    for (int d = 0; d < 3; ++d) st_off(acc + d * n, o, a[d]);
 }
 
 // Scalar-field gather for the interp(F, X) entry point (pm3d.py:91-99).
@@ -130,6 +128,26 @@ __global__ __launch_bounds__(kThreads) void k_gather1(Strided X, long long n, in
 }
 
 // ---------------------------------------------------------------------------------- kick + drift
+// Host scalars of one KDK step (pm3d.py:118-127), with the correctly rounded reciprocals of the
+// three divisors: the device forms each quotient with qdiv, the IEEE quotient of the reference.
+struct KickScalars {
+    double c1p, denp, rdenp;  // pending closing kick of the previous step: p += (c1p*acc)/denp
+    double c1, den1, rden1;   // opening kick: p += (c1*acc)/den1
+    double da, den_d, rden_d; // drift: x += (da*p)/den_d
+};
+__device__ __forceinline__ KickScalars kick_from_tab(const DevStep& e) {
+    return KickScalars{e.c1p, e.denp, e.rdenp, e.c1, e.den1, e.rden1, e.da, e.den_d, e.rden_d};
+}
+
+// Pending closing kick (with `pend`), opening kick and drift of one coordinate, each its own
+// rounding as in pm3d.py:122-127; returns the new position, updates p.
+__device__ __forceinline__ double kick_drift_1(double xk, double& pd, double ak, int pend, const KickScalars& ks,
+                                               double L) {
+    if (pend) pd = pd + qdiv(ks.c1p * ak, ks.denp, ks.rdenp);
+    pd = pd + qdiv(ks.c1 * ak, ks.den1, ks.rden1);
+    return npy_rem_near(xk + qdiv(ks.da * pd, ks.den_d, ks.rden_d), L);
+}
+
 // Opening half kick with the cached acceleration, p += (c1*acc)/den1 (pm3d.py:122), then the drift
 // x = (x + (da*p)/den_d) % L (pm3d.py:123-124).  Non-finite results are counted in *bad.
 // With `pend`, the previous step's deferred closing kick p += (c1p*acc)/denp (pm3d.py:127) is
@@ -138,29 +156,24 @@ __global__ __launch_bounds__(kThreads) void k_gather1(Strided X, long long n, in
 // always applied; past the end of the table the step does nothing.
 __global__ __launch_bounds__(kThreads) void k_kick_drift(double* __restrict__ x, double* __restrict__ p,
                                                          const double* __restrict__ acc, long long n, int pend,
-                                                         double c1p, double denp, double c1, double den1, double da,
-                                                         double den_d, double L, unsigned long long* __restrict__ bad,
+                                                         KickScalars ks, double L, unsigned long long* __restrict__ bad,
                                                          const DevSched* __restrict__ sch, const DevStep* __restrict__ tab) {
     if (sch) {
         const int k = sch->counter;
         if (k >= sch->nsteps) return;
-        const DevStep e = tab[k];
+        ks = kick_from_tab(tab[k]);
         pend = 1;
-        c1p = e.c1p; denp = e.denp; c1 = e.c1; den1 = e.den1; da = e.da; den_d = e.den_d;
     }
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     bool ok = true;
     if (i < n) {
+        const unsigned o = (unsigned)i * 8u;  // n * 8 < 2^32
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const long long k = d * n + i;
-            const double ak = acc[k];
-            double pd = p[k];
-            if (pend) pd = pd + (c1p * ak) / denp;
-            pd = pd + (c1 * ak) / den1;
-            const double xd = npy_rem_near(x[k] + (da * pd) / den_d, L);
-            p[k] = pd;
-            x[k] = xd;
+            double pd = ld_off(p + d * n, o);
+            const double xd = kick_drift_1(ld_off(x + d * n, o), pd, ld_off(acc + d * n, o), pend, ks, L);
+            st_off(p + d * n, o, pd);
+            st_off(x + d * n, o, xd);
             ok &= isfinite(pd) && isfinite(xd);
         }
     }
@@ -177,40 +190,38 @@ __global__ __launch_bounds__(kThreads) void k_kick_drift(double* __restrict__ x,
 // cell is not near its lattice site (the column deposit's test) is appended to its row's list.
 __global__ __launch_bounds__(kThreads) void k_gather_kick_drift(double* __restrict__ x, double* __restrict__ p,
                                                                 long long n, int N, double dx,
-                                                                const double* __restrict__ grad, int pend, double c1p,
-                                                                double denp, double c1, double den1, double da,
-                                                                double den_d, double L,
+                                                                const double* __restrict__ grad, int pend,
+                                                                KickScalars ks, double L,
                                                                 unsigned long long* __restrict__ bad,
                                                                 const DevSched* __restrict__ sch,
-                                                                const DevStep* __restrict__ tab, XMap xm, FarRows fr) {
+                                                                const DevStep* __restrict__ tab, XMap xm, FarRows fr,
+                                                                SiteDiv sdv) {
     if (sch) {
         const int k = sch->counter;
         if (k >= sch->nsteps) return;
-        const DevStep e = tab[k];
+        ks = kick_from_tab(tab[k]);
         pend = 1;
-        c1p = e.c1p; denp = e.denp; c1 = e.c1; den1 = e.den1; da = e.da; den_d = e.den_d;
     }
     const long long i = (long long)xcd_map(blockIdx.x, gridDim.x, xm) * kThreads + threadIdx.x;
     bool ok = true, far = false;
     unsigned row = 0;
     if (i < n) {
-        const double X[3] = {x[i], x[n + i], x[2 * n + i]};
+        const unsigned o = (unsigned)i * 8u;  // n * 8 < 2^32
+        const double X[3] = {ld_off(x, o), ld_off(x + n, o), ld_off(x + 2 * n, o)};
         double a[3], xn[3];
         gather3_at(X, N, dx, grad, a);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const long long k = d * n + i;
-            double pd = p[k];
-            if (pend) pd = pd + (c1p * a[d]) / denp;
-            pd = pd + (c1 * a[d]) / den1;
-            const double xd = npy_rem_near(X[d] + (da * pd) / den_d, L);
-            p[k] = pd;
-            x[k] = xd;
+            double pd = ld_off(p + d * n, o);
+            const double xd = kick_drift_1(X[d], pd, a[d], pend, ks, L);
+            st_off(p + d * n, o, pd);
+            st_off(x + d * n, o, xd);
             xn[d] = xd;
             ok &= isfinite(pd) && isfinite(xd);
         }
         if (fr.cnt) {
-            const int ii = (int)i, s[3] = {ii / (N * N), (ii / N) % N, ii % N};
+            int s[3];
+            site_of((unsigned)i, sdv, s);
             const double inv_dx = 1.0 / dx;
             int lo[3];
             bool nr = true;
@@ -260,11 +271,11 @@ __global__ __launch_bounds__(kThreads) void k_soa_to_aos(const double* __restric
     for (int d = 0; d < 3; ++d) a[i * 3 + d] = s[d * n + i];
 }
 
-// Closing half kick applied lazily before the host reads p: p += (c1*acc)/den.
+// Closing half kick applied lazily before the host reads p: p += (c1*acc)/den (rden = 1/den).
 __global__ __launch_bounds__(kThreads) void k_kick(double* __restrict__ p, const double* __restrict__ acc, long long n3,
-                                                   double c1, double den) {
+                                                   double c1, double den, double rden) {
     const long long k = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (k < n3) p[k] = p[k] + (c1 * acc[k]) / den;
+    if (k < n3) p[k] = p[k] + qdiv(c1 * acc[k], den, rden);
 }
 
 // ---------------------------------------------------------------------------- shell crossing

@@ -33,10 +33,11 @@ struct MxPos {
     const float* off;
     long long n;
     __device__ __forceinline__ void load(long long i, int lo[3], float f[3]) const {
+        const unsigned o = (unsigned)i * 4u;  // n * 4 < 2^32
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            lo[d] = cell[d * n + i];
-            f[d] = off[d * n + i];
+            lo[d] = ld_off(cell + d * n, o);
+            f[d] = ld_off(off + d * n, o);
         }
     }
 };
@@ -71,15 +72,17 @@ __device__ __forceinline__ void gather3_mx_at(const int lo[3], const float f[3],
     const int hx = lo[0] + 1 == N ? 0 : lo[0] + 1, hy = lo[1] + 1 == N ? 0 : lo[1] + 1, hz = lo[2] + 1 == N ? 0 : lo[2] + 1;
     const float wx[2] = {1.0f - f[0], f[0]}, wy[2] = {1.0f - f[1], f[1]}, wz[2] = {1.0f - f[2], f[2]};
     const size_t M = (size_t)N * N * N;
+    const float *g0 = grad, *g1 = grad + M, *g2 = grad + 2 * M;
+    const unsigned uN = (unsigned)N;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int ix = (c >> 2) ? hx : lo[0], iy = ((c >> 1) & 1) ? hy : lo[1], iz = (c & 1) ? hz : lo[2];
-        const float* g = grad + (((size_t)ix * N + iy) * N + iz);
+        const unsigned o = (((unsigned)ix * uN + (unsigned)iy) * uN + (unsigned)iz) * 4u;
         const float w = (wx[c >> 2] * wy[(c >> 1) & 1]) * wz[c & 1];
-        a0 = fmaf(g[0], w, a0);
-        a1 = fmaf(g[M], w, a1);
-        a2 = fmaf(g[2 * M], w, a2);
+        a0 = fmaf(ld_off(g0, o), w, a0);
+        a1 = fmaf(ld_off(g1, o), w, a1);
+        a2 = fmaf(ld_off(g2, o), w, a2);
     }
     a[0] = a0;
     a[1] = a1;
@@ -96,7 +99,7 @@ __device__ __forceinline__ void gather3_mx_body(MxPos P, int N, const float* __r
     P.load(i, lo, f);
     gather3_mx_at(lo, f, N, grad, a);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d * P.n + i] = a[d];
+    for (int d = 0; d < 3; ++d) st_off(acc + d * P.n, (unsigned)i * 4u, a[d]);
 }
 
 __global__ __launch_bounds__(kThreads) void k_gather3_mx(MxPos P, int N, const float* __restrict__ grad,
@@ -124,29 +127,24 @@ __device__ __forceinline__ void kick_drift_mx_body(int* __restrict__ cell, float
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     bool ok = true;
     if (i < n) {
+        const unsigned o = (unsigned)i * 4u;  // n * 4 < 2^32
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const long long k = d * n + i;
-            const float a = acc[k];
-            float pd = p[k];
+            const float a = ld_off(acc + d * n, o);
+            float pd = ld_off(p + d * n, o);
             if (pend) pd = fmaf(kp, a, pd);
             pd = fmaf(k1, a, pd);
-            float f = off[k] + pd * sd;
+            float f = ld_off(off + d * n, o) + pd * sd;
             float fl = floorf(f);
             f -= fl;
             if (f >= 1.0f) {  // a tiny negative f rounds up to 1 after the subtraction: next cell, offset 0
                 f = 0.0f;
                 fl += 1.0f;
             }
-            p[k] = pd;
-            off[k] = f;
+            st_off(p + d * n, o, pd);
+            st_off(off + d * n, o, f);
             ok &= isfinite(pd) && isfinite(f) && fabsf(fl) < 1e6f;
-            if (fl != 0.0f && ok) {
-                int c = cell[k] + (int)fl;
-                c %= N;
-                if (c < 0) c += N;
-                cell[k] = c;
-            }
+            if (fl != 0.0f && ok) st_off(cell + d * n, o, wrap_cell(ld_off(cell + d * n, o) + (int)fl, N));
         }
     }
     const unsigned long long nb = __ballot(!ok);
@@ -172,7 +170,8 @@ __device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell
                                                           const float* __restrict__ grad, int pend, float kp, float k1,
                                                           float sd, unsigned long long* __restrict__ bad,
                                                           const DevSched* __restrict__ sch,
-                                                          const DevStep* __restrict__ tab, XMap xm, FarRows fr) {
+                                                          const DevStep* __restrict__ tab, XMap xm, FarRows fr,
+                                                          SiteDiv sdv) {
     if (sch) {
         const int k = sch->counter;
         if (k >= sch->nsteps) return;
@@ -185,6 +184,7 @@ __device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell
     bool ok = true, far = false;
     unsigned row = 0;
     if (i < n) {
+        const unsigned o = (unsigned)i * 4u;  // n * 4 < 2^32
         int lo[3];
         float f[3], a[3];
         MxPos{cell, off, n}.load(i, lo, f);
@@ -192,11 +192,11 @@ __device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell
         int nc[3] = {lo[0], lo[1], lo[2]};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const long long k = d * n + i;
+            float* pk = (float*)((char*)(p + d * n) + o);
             // p and the new offsets are streamed with non-temporal loads/stores: they are not read
             // again before the next step, and leaving them out of the caches keeps more of the
             // positions there for the deposit that follows (mixed 256^3: deposit 152 -> 142 us).
-            float pd = __builtin_nontemporal_load(p + k);
+            float pd = __builtin_nontemporal_load(pk);
             if (pend) pd = fmaf(kp, a[d], pd);
             pd = fmaf(k1, a[d], pd);
             float fo = f[d] + pd * sd;
@@ -206,20 +206,19 @@ __device__ __forceinline__ void gather_kick_drift_mx_body(int* __restrict__ cell
                 fo = 0.0f;
                 fl += 1.0f;
             }
-            __builtin_nontemporal_store(pd, p + k);
-            __builtin_nontemporal_store(fo, off + k);
+            __builtin_nontemporal_store(pd, pk);
+            __builtin_nontemporal_store(fo, (float*)((char*)(off + d * n) + o));
             ok &= isfinite(pd) && isfinite(fo) && fabsf(fl) < 1e6f;
             if (fl != 0.0f && ok) {
-                int c = lo[d] + (int)fl;
-                c %= N;
-                if (c < 0) c += N;
-                cell[k] = c;
+                const int c = wrap_cell(lo[d] + (int)fl, N);
+                st_off(cell + d * n, o, c);
                 nc[d] = c;
             }
         }
         if (fr.cnt) {
-            const int ii = (int)i;
-            far = !(near_cell(nc[0], N, ii / (N * N)) && near_cell(nc[1], N, (ii / N) % N) && near_cell(nc[2], N, ii % N));
+            int st[3];
+            site_of((unsigned)i, sdv, st);
+            far = !(near_cell(nc[0], N, st[0]) && near_cell(nc[1], N, st[1]) && near_cell(nc[2], N, st[2]));
             const bool inr = (unsigned)nc[0] < (unsigned)N && (unsigned)nc[1] < (unsigned)N;
             row = inr ? (unsigned)(nc[0] * N + nc[1]) : 0u;
         }
@@ -235,8 +234,9 @@ __global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx(int* __restri
                                                                    float k1, float sd,
                                                                    unsigned long long* __restrict__ bad,
                                                                    const DevSched* __restrict__ sch,
-                                                                   const DevStep* __restrict__ tab, XMap xm, FarRows fr) {
-    gather_kick_drift_mx_body(cell, off, p, n, N, grad, pend, kp, k1, sd, bad, sch, tab, xm, fr);
+                                                                   const DevStep* __restrict__ tab, XMap xm, FarRows fr,
+                                                                   SiteDiv sdv) {
+    gather_kick_drift_mx_body(cell, off, p, n, N, grad, pend, kp, k1, sd, bad, sch, tab, xm, fr, sdv);
 }
 
 __global__ __launch_bounds__(kThreads) void k_kick_mx(float* __restrict__ p, const float* __restrict__ acc, long long n3,
@@ -352,6 +352,90 @@ __device__ __forceinline__ void min_jacobian_mx_body(MxPos P, int Np, int N, dou
 __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, int N, double inv_c3,
                                                               double* __restrict__ bmin) {
     min_jacobian_mx_body(P, Np, N, inv_c3, bmin);
+}
+
+// The same minimum with four consecutive lattice sites (one z row, Np % 4 == 0) per thread: the
+// cell and offset words of the four sites and of their +x and +y neighbours are one 16-byte load
+// per array (18 loads for four sites instead of 96 4-byte loads), the +z neighbour of a site is
+// the next site of the four, and only the fourth one's is loaded on its own.  Each site's
+// determinant is the arithmetic of min_jacobian_mx_body; the per-thread minimum is taken on the
+// ordered bits, like block_min.  Block b holds the minimum of sites [1024 b, 1024 b + 1024).
+__device__ __forceinline__ double det_mx(const int ci[3], const float fi[3], const int cn[3][3], const float fn[3][3],
+                                         int N, double inv_c3) {
+    double J[3][3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int dc = cn[e][d] - ci[d];
+            if (dc > N / 2) dc -= N;
+            if (dc < -N / 2) dc += N;
+            J[e][d] = (double)dc + ((double)fn[e][d] - (double)fi[d]);
+        }
+    }
+    return (J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) * inv_c3;
+}
+
+__device__ __forceinline__ void min_jacobian_mx4_body(MxPos P, int N, double inv_c3, SiteDiv sdv,
+                                                      double* __restrict__ bmin) {
+    const unsigned n = (unsigned)P.n, np = sdv.Np;
+    const unsigned i = 4u * (blockIdx.x * kThreads + threadIdx.x);
+    long long ob = ordered_bits(1e300);
+    if (i < n) {
+        int st[3];
+        site_of(i, sdv, st);
+        const unsigned a = (unsigned)st[0], b = (unsigned)st[1], c = (unsigned)st[2];
+        const unsigned ixn = i + ((a + 1 == np ? 0u : a + 1) - a) * np * np;  // +x neighbour of site i
+        const unsigned iyn = i + ((b + 1 == np ? 0u : b + 1) - b) * np;       // +y neighbour
+        const unsigned izn = i - c + (c + 4 == np ? 0u : c + 4);              // +z neighbour of site i + 3
+        int4 C[3], CX[3], CY[3];
+        float4 F[3], FX[3], FY[3];
+        int cz[3];
+        float fz[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int4* cb = (const int4*)(P.cell + (size_t)d * n);
+            const float4* fb = (const float4*)(P.off + (size_t)d * n);
+            C[d] = ld_off(cb, i * 4u);
+            F[d] = ld_off(fb, i * 4u);
+            CX[d] = ld_off(cb, ixn * 4u);
+            FX[d] = ld_off(fb, ixn * 4u);
+            CY[d] = ld_off(cb, iyn * 4u);
+            FY[d] = ld_off(fb, iyn * 4u);
+            cz[d] = ld_off(P.cell + (size_t)d * n, izn * 4u);
+            fz[d] = ld_off(P.off + (size_t)d * n, izn * 4u);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int ci[3], cn[3][3];
+            float fi[3], fn[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int cv[4] = {C[d].x, C[d].y, C[d].z, C[d].w}, cxv[4] = {CX[d].x, CX[d].y, CX[d].z, CX[d].w},
+                          cyv[4] = {CY[d].x, CY[d].y, CY[d].z, CY[d].w};
+                const float fv[4] = {F[d].x, F[d].y, F[d].z, F[d].w}, fxv[4] = {FX[d].x, FX[d].y, FX[d].z, FX[d].w},
+                            fyv[4] = {FY[d].x, FY[d].y, FY[d].z, FY[d].w};
+                ci[d] = cv[k];
+                fi[d] = fv[k];
+                cn[0][d] = cxv[k];
+                fn[0][d] = fxv[k];
+                cn[1][d] = cyv[k];
+                fn[1][d] = fyv[k];
+                cn[2][d] = k < 3 ? cv[k + 1 < 4 ? k + 1 : 3] : cz[d];
+                fn[2][d] = k < 3 ? fv[k + 1 < 4 ? k + 1 : 3] : fz[d];
+            }
+            const long long o = ordered_bits(det_mx(ci, fi, cn, fn, N, inv_c3));
+            ob = (k == 0 || o < ob) ? o : ob;
+        }
+    }
+    const double m = block_min(from_ordered_bits(ob));
+    if (threadIdx.x == 0) bmin[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(kThreads) void k_min_jacobian_mx4(MxPos P, int N, double inv_c3, SiteDiv sdv,
+                                                               double* __restrict__ bmin) {
+    min_jacobian_mx4_body(P, N, inv_c3, sdv, bmin);
 }
 
 }  // namespace pm3d

@@ -902,6 +902,13 @@ int tune_deposit(Handle* h) {
     return live_init(h, all ? every : live ? cand : 0u);
 }
 
+// The FP64 step scalars with the reciprocals of their divisors (qdiv), from the host scalars of
+// pm3d.py:118-127 and the pending closing kick of the previous step.
+KickScalars kick_scalars(const Handle* h, const pm3d_step_scalars* sc) {
+    const double dp = h->kick_pending ? h->pend_den : 1.0;
+    return KickScalars{h->pend_c1, dp, 1.0 / dp, sc->c1, sc->den1, 1.0 / sc->den1, sc->da, sc->den_d, 1.0 / sc->den_d};
+}
+
 // 1/c^3 for the mixed Jacobian, c = N/Np cells per lattice site
 double inv_c3(const Handle* h) {
     const double c = (double)h->N / h->Np;
@@ -934,8 +941,8 @@ int gather_state(Handle* h, void* acc_out) {
     {
         Timed t(h, KC_GATHER);
         if (!h->mixed)
-            k_gather3<false><<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, (const double*)h->grad,
-                                                                     (double*)acc_out, h->p, 0, 1, gather_map(h));
+            k_gather3<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->n, h->N, h->dx, (const double*)h->grad,
+                                                              (double*)acc_out, gather_map(h));
         else
             k_gather3_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(mxpos(h), h->N, (const float*)h->grad, (float*)acc_out,
                                                                gather_map(h));
@@ -964,7 +971,8 @@ int flush_kick(Handle* h) {
     if (int rc = settle_acc(h)) return rc;
     Timed t(h, KC_OTHER);
     if (!h->mixed)
-        k_kick<<<grid_for(3 * h->n), kThreads, 0, h->st>>>(h->p, h->acc, 3 * h->n, h->pend_c1, h->pend_den);
+        k_kick<<<grid_for(3 * h->n), kThreads, 0, h->st>>>(h->p, h->acc, 3 * h->n, h->pend_c1, h->pend_den,
+                                                            1.0 / h->pend_den);
     else
         k_kick_mx<<<grid_for(3 * h->n), kThreads, 0, h->st>>>(h->pf, h->accf, 3 * h->n, (float)(h->pend_c1 / h->pend_den));
     h->kick_pending = false;
@@ -1103,6 +1111,24 @@ void spec_amp_n(Handle* h, double kcut) {
                                                                                        1.0 / ((double)N * h->dx), kcut);
 }
 
+// Per-block minima of det(dx/dq) into h->bmax; returns the number of blocks.  Mixed lattices with
+// Np % 4 == 0 take four sites per thread (k_min_jacobian_mx4).
+unsigned launch_min_jacobian(Handle* h) {
+    if (!h->mixed) {
+        const unsigned g = grid_for(h->n);
+        k_min_jacobian<<<g, kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, h->bmax);
+        return g;
+    }
+    if (h->Np % 4 == 0) {
+        const unsigned g = grid_for(h->n / 4);
+        k_min_jacobian_mx4<<<g, kThreads, 0, h->st>>>(mxpos(h), h->N, inv_c3(h), site_div(h->Np), h->bmax);
+        return g;
+    }
+    const unsigned g = grid_for(h->n);
+    k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, inv_c3(h), h->bmax);
+    return g;
+}
+
 // One step driven by the device schedule (for graph capture): the previous step's gather fused with
 // the kicks and drift of the scheduled scalars, force meshes, Jacobian minimum, schedule advance.
 // Launches only; no host synchronisation.
@@ -1110,22 +1136,20 @@ int launch_sched_step(Handle* h) {
     const FarRows fr = far_rows(h);
     if (!h->mixed)
         k_gather_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->n, h->N, h->dx, (const double*)h->grad,
-                                                                    1, 0, 1, 0, 1, 0, 1, h->L, h->dscal + 1, h->sched,
-                                                                    h->tab, gather_map(h), fr);
+                                                                    1, KickScalars{0, 1, 1, 0, 1, 1, 0, 1, 1}, h->L,
+                                                                    h->dscal + 1, h->sched, h->tab, gather_map(h), fr,
+                                                                    site_div(h->Np));
     else
         k_gather_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(h->cell, h->off, h->pf, h->n, h->N,
                                                                        (const float*)h->grad, 1, 0, 0, 0, h->dscal + 1,
-                                                                       h->sched, h->tab, gather_map(h), fr);
+                                                                       h->sched, h->tab, gather_map(h), fr,
+                                                                       site_div(h->Np));
     h->lists_fresh = fr.cnt != nullptr;
     h->use_sched = true;
     int rc = solve_state(h, 0.0, 0.0);
     h->use_sched = false;
     if (rc) return rc;
-    const unsigned g = grid_for(h->n);
-    if (!h->mixed)
-        k_min_jacobian<<<g, kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, h->bmax);
-    else
-        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, inv_c3(h), h->bmax);
+    const unsigned g = launch_min_jacobian(h);
     k_sched_advance<<<1, kThreads, 0, h->st>>>(h->sched, h->bmax, (int)g, h->jhist);
     return check_launch(h, "scheduled step");
 }
@@ -1179,6 +1203,7 @@ MxBox box_of(Handle* h) {
     b.qslot = qmax_slot(h);
     b.kval = 1.0 / ((double)h->N * h->dx);
     b.inv_c3 = inv_c3(h);
+    b.sdv = site_div(h->Np);
     b.qscale = h->qscale;
     b.nmean = h->nmean;
     (void)specN;
@@ -1233,8 +1258,9 @@ void launch_boxes_n(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
         constexpr int Q = Fft<float, N>::kInvQ;
         k_inv_plane_boxes<N, Q><<<dim3(3 * Q * N, nb), InvTile<float, N, Q>::NT, 0, st>>>(dev, xmap(3 * Q * N, Q));
     }
-    k_min_jacobian_mx_boxes<<<dim3(g, nb), kThreads, 0, st>>>(dev);
-    k_sched_advance_boxes<<<dim3(1, nb), kThreads, 0, st>>>(dev, (int)g);
+    const unsigned g4 = grid_for(h0->n / 4);  // Np = N, a multiple of 4
+    k_min_jacobian_mx_boxes<<<dim3(g4, nb), kThreads, 0, st>>>(dev);
+    k_sched_advance_boxes<<<dim3(1, nb), kThreads, 0, st>>>(dev, (int)g4);
 }
 
 void launch_boxes(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
@@ -1351,6 +1377,8 @@ __global__ __launch_bounds__(kThreads) void k_probe_read(const probe_v4* __restr
 // ================================================================================ C-ABI
 extern "C" {
 
+constexpr int kMaxNp = 812;
+
 int pm3d_create(const pm3d_cfg* cfg, void** handle) {
     if (!cfg || !handle) {
         g_create_error = "null argument";
@@ -1364,8 +1392,10 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
         g_create_error = "N must be a power of two in [8,512], or 48 or 96";
         return PM3D_EUNSUPPORTED;
     }
-    if (cfg->Np < 1 || cfg->Np > 1024 || !(cfg->L > 0.0) || !(cfg->smooth >= 0.0)) {
-        g_create_error = "invalid Np, L or smooth";
+    // Np <= 812: every per-particle array of 8-byte elements stays under 4 GiB, so the particle
+    // kernels address it with 32-bit byte offsets (n * 8 < 2^32)
+    if (cfg->Np < 1 || cfg->Np > kMaxNp || !(cfg->L > 0.0) || !(cfg->smooth >= 0.0)) {
+        g_create_error = "invalid Np (1 .. 812), L or smooth";
         return PM3D_EINVAL;
     }
     if (cfg->precision != PM3D_FP64 && cfg->precision != PM3D_MIXED) {
@@ -1767,21 +1797,22 @@ int pm3d_step(void* handle, const pm3d_step_scalars* sc, double* delta, double* 
         const FarRows fr = far_rows(h);
         if (!h->mixed)
             t.launch(k_gather_kick_drift, grid_for(h->n), kThreads, h->x, h->p, h->n, h->N, h->dx,
-                     (const double*)h->grad, h->kick_pending ? 1 : 0, h->pend_c1, h->pend_den, sc->c1, sc->den1, sc->da,
-                     sc->den_d, h->L, h->dscal + 1, nullptr, nullptr, gather_map(h), fr);
+                     (const double*)h->grad, h->kick_pending ? 1 : 0, kick_scalars(h, sc), h->L, h->dscal + 1, nullptr,
+                     nullptr, gather_map(h), fr, site_div(h->Np));
         else
             t.launch(k_gather_kick_drift_mx, grid_for(h->n), kThreads, h->cell, h->off, h->pf, h->n, h->N,
                      (const float*)h->grad, h->kick_pending ? 1 : 0,
                      (float)(h->kick_pending ? h->pend_c1 / h->pend_den : 0.0), (float)(sc->c1 / sc->den1),
-                     (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr, gather_map(h), fr);
+                     (float)(sc->da / (sc->den_d * h->dx)), h->dscal + 1, nullptr, nullptr, gather_map(h), fr,
+                     site_div(h->Np));
         h->lists_fresh = fr.cnt != nullptr;
     } else {
         // the stored acceleration (a state whose force meshes were replaced after it was gathered)
         Timed t(h, KC_KICK_DRIFT);
         if (!h->mixed)
             k_kick_drift<<<grid_for(h->n), kThreads, 0, h->st>>>(h->x, h->p, h->acc, h->n, h->kick_pending ? 1 : 0,
-                                                                 h->pend_c1, h->pend_den, sc->c1, sc->den1, sc->da,
-                                                                 sc->den_d, h->L, h->dscal + 1, nullptr, nullptr);
+                                                                 kick_scalars(h, sc), h->L, h->dscal + 1, nullptr,
+                                                                 nullptr);
         else
             k_kick_drift_mx<<<grid_for(h->n), kThreads, 0, h->st>>>(
                 h->cell, h->off, h->pf, h->accf, h->n, h->N, h->kick_pending ? 1 : 0,
@@ -1836,11 +1867,7 @@ int pm3d_min_jacobian(void* handle, double* jmin) {
     if (!handle || !jmin) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
     if (!h->state_set) return fail(h, PM3D_ESTATE, "state not set");
-    const unsigned g = grid_for(h->n);
-    if (!h->mixed)
-        k_min_jacobian<<<g, kThreads, 0, h->st>>>(h->x, h->Np, h->L, h->L / h->Np, h->bmax);
-    else
-        k_min_jacobian_mx<<<g, kThreads, 0, h->st>>>(mxpos(h), h->Np, h->N, inv_c3(h), h->bmax);
+    const unsigned g = launch_min_jacobian(h);
     std::vector<double> hb(g);
     HCHK(h, hipMemcpyAsync(hb.data(), h->bmax, g * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
@@ -2188,6 +2215,7 @@ int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps) {
         DevStep& d = h->htab[k];
         d.c1p = e.c1p; d.denp = e.denp; d.c1 = e.c1; d.den1 = e.den1; d.da = e.da; d.den_d = e.den_d;
         d.den2 = e.den2; d.s = e.s; d.sm2 = e.sm2;
+        d.rdenp = 1.0 / e.denp; d.rden1 = 1.0 / e.den1; d.rden_d = 1.0 / e.den_d;
         d.kp = (float)(e.c1p / e.denp);
         d.k1 = (float)(e.c1 / e.den1);
         d.sd = (float)(e.da / (e.den_d * h->dx));

@@ -50,6 +50,7 @@ class Request:
     t_done: float = float("nan")
     result: object = None
     error: str | None = None
+    rejected: bool = False  # invalid parameters, turned away at admission (no slot, no device work)
 
     @property
     def latency(self):
@@ -194,6 +195,7 @@ class CollapseServer:
             ic.validate(spec)  # an invalid request fails here, before any slot or device work
         except ValueError as err:
             req.t_admit = now
+            req.rejected = True
             self._fail(req, None, f"ValueError: {err}", now)
             return True
         eng = self._engine(spec.N)
@@ -350,13 +352,17 @@ def config_requests(n: int, N: int = 256, seed0: int = 1000, noise: float = 0.01
 
 
 def summarize(done, t_wall=None):
+    """completed: requests with a result; rejected: requests whose parameters describe no peak
+    (turned away at admission); failed: requests that errored after admission."""
     ok = [r for r in done if r.error is None]
+    rejected = sum(1 for r in done if r.error is not None and r.rejected)
     lat = np.array([r.latency for r in ok]) if ok else np.array([np.nan])
     t_end = max(r.t_done for r in done)
     t_first = min(r.arrival for r in done)
     return {
         "completed": len(ok),
-        "failed": len(done) - len(ok),
+        "rejected_invalid": rejected,
+        "failed": len(done) - len(ok) - rejected,
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "mean_latency_s": float(lat.mean()),

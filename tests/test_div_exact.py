@@ -51,3 +51,92 @@ def test_fma_division_is_ieee_quotient(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     tot, bad = map(int, out.stdout.split())
     assert tot == 7 * 9 * 300000 and bad == 0
+
+
+KICK_SRC = r"""
+#include <stdio.h>
+#include <math.h>
+#include <stdint.h>
+static uint64_t s = 0x9E3779B97F4A7C15ULL;
+static double urand(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return (s >> 11) * 0x1p-53; }
+int main(void) {
+  /* the divisors of the KDK schedule (pm3d.py:118-127): a*H = H_I a^(-1/2), a_mid^3 H_mid =
+     H_I a_mid^(3/2), for a from 1e-3 to past 1e3; numerators of every sign and a wide exponent range */
+  long bad = 0, tot = 0;
+  for (int j = 0; j < 1500; ++j) {
+    const double a = 1e-3 * pow(1.02, j * 0.47), am = a + 0.5 * 0.0125 * a;
+    const double ys[3] = {a * (5.0 * pow(a, -1.5)), pow(am, 3) * (5.0 * pow(am, -1.5)), (a + 0.0125 * a) * (5.0 * pow(a + 0.0125 * a, -1.5))};
+    for (int t = 0; t < 3; ++t) {
+      const double y = ys[t], r = 1.0 / y;
+      for (int k = 0; k < 4000; ++k) {
+        double x = (urand() - 0.5) * ldexp(1.0, (int)(urand() * 120) - 90);
+        if (k % 5 == 0) x = nextafter(x, -x);
+        const double q0 = x * r;
+        const double q = x == 0.0 ? q0 : fma(fma(-q0, y, x), r, q0);
+        if (q != x / y) ++bad;
+        ++tot;
+      }
+    }
+  }
+  printf("%ld %ld\n", tot, bad);
+  return bad != 0;
+}
+"""
+
+
+def test_reciprocal_division_of_kick_scalars_is_ieee_quotient(tmp_path):
+    """The particle kernels' qdiv (csrc/common.hpp): x/y as q0 = x*r, r = 1/y correctly rounded on the
+    host, plus one Markstein correction, bit for bit the IEEE quotient for the step's divisors."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "kdiv.c"
+    src.write_text(KICK_SRC)
+    exe = tmp_path / "kdiv"
+    subprocess.run([gcc, "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    tot, bad = map(int, out.stdout.split())
+    assert tot == 1500 * 3 * 4000 and bad == 0
+
+
+MAGIC_SRC = r"""
+#include <stdio.h>
+#include <stdint.h>
+/* csrc/common.hpp div32_make / div32: q = (umulhi(n, m) + n) >> s */
+static void make(uint32_t d, uint32_t* m, uint32_t* sh) {
+  uint32_t s = 0; while ((1ull << s) < d) ++s;
+  *m = (uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1); *sh = s;
+}
+int main(void) {
+  const uint32_t Ns[] = {1, 2, 3, 8, 16, 32, 48, 64, 96, 128, 256, 512, 812};
+  long bad = 0, tot = 0;
+  for (int k = 0; k < 13; ++k)
+    for (int sq = 0; sq < 2; ++sq) {
+      const uint32_t d = sq ? Ns[k] * Ns[k] : Ns[k];
+      uint32_t m, sh; make(d, &m, &sh);
+      for (uint64_t n = 0; n < (1ull << 31); n += (n < (1u << 22) ? 1 : 997)) {
+        const uint32_t q = (uint32_t)((((uint64_t)n * m) >> 32) + n) >> sh;
+        if (q != n / d) ++bad;
+        ++tot;
+      }
+    }
+  printf("%ld %ld\n", tot, bad);
+  return bad != 0;
+}
+"""
+
+
+def test_magic_division_of_lattice_sites(tmp_path):
+    """div32 (csrc/common.hpp), the particle index -> lattice site division of the particle kernels,
+    equals integer division for every divisor Np and Np^2 of the supported sizes, on all indices
+    below 2^22 and a stride of the rest of [0, 2^31)."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "magic.c"
+    src.write_text(MAGIC_SRC)
+    exe = tmp_path / "magic"
+    subprocess.run([gcc, "-O2", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    tot, bad = map(int, out.stdout.split())
+    assert tot > 0 and bad == 0

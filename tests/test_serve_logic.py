@@ -164,7 +164,7 @@ def test_failed_request_does_not_disturb_the_others():
     # the two failed slots were released; the three good ones are idle, plus the engine
     assert StubPM.live == 3 + 1
     s = summarize(list(done.values()))
-    assert s["completed"] == 3 and s["failed"] == 2
+    assert s["completed"] == 3 and s["failed"] == 2 and s["rejected_invalid"] == 0
 
 
 @pytest.mark.timeout(30)
@@ -203,3 +203,6 @@ def test_invalid_request_fails_before_any_slot():
     assert "Zel'dovich axes undefined" in done[1].error and done[1].result is None
     assert done[2].error is None and done[2].result.crossed
     assert StubPM.live == 2  # one slot and its engine: the invalid request allocated nothing
+    assert done[1].rejected and not done[2].rejected
+    s = summarize(list(done.values()))
+    assert (s["completed"], s["rejected_invalid"], s["failed"]) == (1, 1, 0)
