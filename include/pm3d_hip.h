@@ -253,6 +253,18 @@ int pm3d_batch_set(void* batch, void* const* handles, int count); /* (re)capture
 int pm3d_batch_run(void* batch, int nsteps);                      /* replay nsteps times, asynchronous */
 int pm3d_batch_synchronize(void* batch);
 
+/* Schedule status of every member of the batch, in the order of pm3d_batch_set, read in one
+ * device pass after the replays queued so far (one synchronisation for the whole batch instead of
+ * two copies per member).  No counterpart in the reference: the reference runs one box per
+ * process (pbhgr/scripts/pm3d_peak.py:24-33) and its campaign driver polls process results
+ * (pbhgr/pbhgr/scan.py:67-76). */
+typedef struct {
+    int32_t done;   /* scheduled steps taken */
+    int32_t nsteps; /* entries of the table; set to done when a step ends at the first shell crossing */
+    double jmin;    /* min det(dx/dq) after the last step taken (1e300 before the first) */
+} pm3d_member_status;
+int pm3d_batch_status(void* batch, pm3d_member_status* out, int count); /* count = members of the batch */
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,10 @@ class Pm3dSchedEntry(ctypes.Structure):
                 ("da", c_double), ("den_d", c_double), ("den2", c_double), ("s", c_double), ("sm2", c_double)]
 
 
+class Pm3dMemberStatus(ctypes.Structure):
+    _fields_ = [("done", c_int32), ("nsteps", c_int32), ("jmin", c_double)]
+
+
 PD = POINTER(c_double)
 PI64 = POINTER(c_int64)
 
@@ -91,6 +95,7 @@ SIGNATURES = {
     "pm3d_batch_set": (c_int, [c_void_p, POINTER(c_void_p), c_int]),
     "pm3d_batch_run": (c_int, [c_void_p, c_int]),
     "pm3d_batch_synchronize": (c_int, [c_void_p]),
+    "pm3d_batch_status": (c_int, [c_void_p, POINTER(Pm3dMemberStatus), c_int]),
 }
 
 _lib = None

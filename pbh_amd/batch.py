@@ -5,8 +5,9 @@ a multiprocessing Pool over spherical runs); pm3d itself runs one box per proces
 (pbhgr/scripts/pm3d_peak.py).  Here every box (PM3D handle, any mesh size, FP64 or mixed) keeps its
 state in HBM with a device-resident step schedule, and one captured hipGraph holds one step of every
 active box as independent branches; a replay advances all of them.  Each box records min det(dx/dq)
-per step on the device; the host polls the histories every `poll` replays, retires boxes that
-crossed (or ran out of schedule) and re-captures the graph for the rest.
+per step on the device and ends its own schedule at its first crossing; every `poll` replays the host
+reads the status of all boxes in one device pass, retires those whose schedule ended and re-captures
+the graph for the rest.
 
 A box gives bitwise the same trajectory in a batch as alone: the kernels and the host scalars are
 the same (tests/test_gpu_batch.py).
@@ -63,6 +64,12 @@ class Batch:
     def synchronize(self):
         self._chk(self._L.pm3d_batch_synchronize(self._b))
 
+    def status(self, count: int):
+        """(done, nsteps, jmin) of every member, in the order of set(), after the queued replays."""
+        out = (_lib.Pm3dMemberStatus * max(count, 1))()
+        self._chk(self._L.pm3d_batch_status(self._b, out, count))
+        return [(out[i].done, out[i].nsteps, out[i].jmin) for i in range(count)]
+
     def __del__(self):
         b = getattr(self, "_b", None)
         if b is not None and b.value:
@@ -89,54 +96,59 @@ def _finish(m: Member, crossed_at: int | None):
 def run_batched(members: list[Member], max_steps: int = 4000, poll: int = 32, batch: Batch | None = None):
     """Step every member to its first shell crossing (or a_max) in one graph-replayed batch.  A member's
     schedule ends on the device at its crossing (its later steps do nothing), so `poll` replays run
-    between host status reads without wasted work."""
+    between host status reads without wasted work; a status read is one device pass over all members
+    (Batch.status), and a member's history is read once, when its table has ended."""
     batch = batch or Batch()
     t0 = time.perf_counter()
-    active = []
+    # the first (plain) step of every member is queued before any result is read, so the members'
+    # streams run their first steps side by side
     for m in members:
         m.j0 = m.pm.min_jacobian()
+    for m in members:
         da = min(m.da_frac * m.a0, m.a_max - m.a0)
         a1, _ = m.pm.step(m.a0, da)
         m.a_vals = [a1]
+    active = []
+    for m in members:
         m.jhist = [m.pm.min_jacobian()]
         if m.jhist[0] <= 0.0:
             _finish(m, 0)
             continue
-        ents, avals = m.pm.schedule_entries(a1, m.da_frac, m.a_max, max_steps, m.pm.last_kick)
+        ents, avals = m.pm.schedule_entries(m.a_vals[0], m.da_frac, m.a_max, max_steps, m.pm.last_kick)
         if not avals:
             _finish(m, None)
             continue
         m.pm.set_schedule(ents, len(avals))
         m._sched_a = avals
         active.append(m)
-    batch.set([m.pm for m in active])
-    built = len(active)  # members in the captured graph
+    graph = list(active)  # members of the captured graph, in its order
+    batch.set([m.pm for m in graph])
     while active:
         batch.run(poll)
-        batch.synchronize()
+        st = batch.status(len(graph))
+        where = {id(m): k for k, m in enumerate(graph)}
         still = []
         for m in active:
+            done, nsteps, _ = st[where[id(m)]]
+            if done < nsteps:
+                still.append(m)
+                continue
+            # the table ended: at the first crossing (the device shortened it) or at its last entry
             done, hist = m.pm.schedule_status()
             m.jhist = [m.jhist[0]] + list(hist)
             m.a_vals = [m.a_vals[0]] + m._sched_a[:done]
             neg = np.nonzero(np.asarray(m.jhist) <= 0.0)[0]
-            if len(neg):
-                _finish(m, int(neg[0]))
-                m.t_done = time.perf_counter() - t0
-            elif done >= len(m._sched_a):
-                _finish(m, None)
-                m.t_done = time.perf_counter() - t0
-            else:
-                still.append(m)
+            _finish(m, int(neg[0]) if len(neg) else None)
+            m.t_done = time.perf_counter() - t0
         # a finished member's steps do nothing on the device (its schedule ends at the crossing), so
         # the graph is rebuilt without them only once a quarter of its members have finished
-        if still and built - len(still) >= max(1, built // 4):
-            batch.set([m.pm for m in still])
-            built = len(still)
+        if still and len(graph) - len(still) >= max(1, len(graph) // 4):
+            graph = list(still)
+            batch.set([m.pm for m in graph])
         active = still
     for m in members:
         if getattr(m, "_sched_a", None) is not None:
-            done = m.pm.schedule_end(m.a_vals[-1])
+            m.pm.schedule_end(m.a_vals[-1])
     return [m.result for m in members]
 
 

@@ -1154,8 +1154,28 @@ int launch_sched_step(Handle* h) {
     return check_launch(h, "scheduled step");
 }
 
+// Per-member pointers of a batch's status pass (k_batch_status).
+struct BatchRef {
+    const DevSched* sched;
+    const double* jhist;
+};
+__global__ __launch_bounds__(kThreads) void k_batch_status(const BatchRef* __restrict__ refs, int count,
+                                                           pm3d_member_status* __restrict__ out) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= count) return;
+    const DevSched s = *refs[i].sched;
+    pm3d_member_status o;
+    o.done = s.counter;
+    o.nsteps = s.nsteps;
+    o.jmin = s.counter > 0 ? refs[i].jhist[s.counter - 1] : 1e300;
+    out[i] = o;
+}
+
 struct Batch {
     hipStream_t st = nullptr;
+    BatchRef* refs = nullptr;             // device, one per member
+    pm3d_member_status* stat = nullptr;   // pinned host memory the status pass writes
+    int ref_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t fork = nullptr;
@@ -2282,6 +2302,8 @@ int pm3d_batch_destroy(void* batch) {
     hipStreamSynchronize(b->st);
     batch_clear_graph(b);
     batch_free_boxes(b);
+    if (b->refs) hipFree(b->refs);
+    if (b->stat) hipHostFree(b->stat);
     for (auto e : b->joins) hipEventDestroy(e);
     for (auto e : b->cls_join) hipEventDestroy(e);
     for (auto cs : b->cls_st) hipStreamDestroy(cs);
@@ -2313,6 +2335,27 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
         b->members.push_back(h);
     }
     if (count == 0) return 0;
+    if (count > b->ref_cap) {
+        if (b->refs) hipFree(b->refs);
+        if (b->stat) hipHostFree(b->stat);
+        b->refs = nullptr;
+        b->stat = nullptr;
+        b->ref_cap = 0;
+        if (hipMalloc((void**)&b->refs, count * sizeof(BatchRef)) != hipSuccess ||
+            hipHostMalloc((void**)&b->stat, count * sizeof(pm3d_member_status), hipHostMallocDefault) != hipSuccess) {
+            b->err = "allocation of the batch status buffers failed";
+            return PM3D_ENOMEM;
+        }
+        b->ref_cap = count;
+    }
+    {
+        std::vector<BatchRef> hr(count);
+        for (int i = 0; i < count; ++i) hr[i] = BatchRef{b->members[i]->sched, b->members[i]->jhist};
+        if (hipMemcpy(b->refs, hr.data(), count * sizeof(BatchRef), hipMemcpyHostToDevice) != hipSuccess) {
+            b->err = "copy of the batch status pointers failed";
+            return PM3D_EHIP;
+        }
+    }
     while ((int)b->joins.size() < count) {
         hipEvent_t e;
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
@@ -2425,6 +2468,25 @@ int pm3d_batch_run(void* batch, int nsteps) {
             return PM3D_EHIP;
         }
     }
+    return 0;
+}
+
+int pm3d_batch_status(void* batch, pm3d_member_status* out, int count) {
+    if (!batch || !out || count < 0) return PM3D_EINVAL;
+    Batch* b = (Batch*)batch;
+    if (count != (int)b->members.size()) {
+        b->err = "pm3d_batch_status: count differs from the members of pm3d_batch_set";
+        return PM3D_EINVAL;
+    }
+    if (count == 0) return 0;
+    k_batch_status<<<grid_for(count), kThreads, 0, b->st>>>(b->refs, count, b->stat);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(b->st);
+    if (e != hipSuccess) {
+        b->err = std::string("batch status: ") + hipGetErrorString(e);
+        return PM3D_EHIP;
+    }
+    std::memcpy(out, b->stat, count * sizeof(pm3d_member_status));
     return 0;
 }
 

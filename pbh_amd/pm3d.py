@@ -16,6 +16,7 @@ Every compute entry point goes through `_pm3d_hip.so`; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import functools
 
 import numpy as np
 
@@ -55,6 +56,23 @@ def step_scalars(a: float, da: float, smooth: float, dx: float, need_forces: boo
     sc.sm2 = (smooth * dx) ** 2 if smooth > 0 else 0.0
     sc.need_forces = 1 if need_forces else 0
     return sc
+
+
+@functools.lru_cache(maxsize=256)
+def _schedule_table(a_start, da_frac, a_end, max_steps, c1p, denp, smooth, dx):
+    """The schedule table of PM3D.schedule_entries: host scalars of every step (step_scalars, in
+    the order of pm3d.py:118-127) until a_end or max_steps."""
+    rows, avals = [], []
+    a = a_start
+    while len(avals) < max_steps and a < a_end:
+        da = min(da_frac * a, a_end - a)
+        sc = step_scalars(a, da, smooth, dx, False)
+        rows.append((c1p, denp, sc.c1, sc.den1, sc.da, sc.den_d, sc.den2, sc.s_new, sc.sm2))
+        c1p, denp = sc.c1, sc.den2
+        a = a + da
+        avals.append(a)
+    ents = (_lib.Pm3dSchedEntry * max(len(rows), 1))(*rows)
+    return ents, tuple(avals)
 
 
 class DeltaMesh:
@@ -310,23 +328,11 @@ class PM3D:
     def schedule_entries(self, a_start: float, da_frac: float, a_end: float, max_steps: int, prev: tuple):
         """Schedule table continuing from a_start with da = min(da_frac*a, a_end-a) (pm3d_peak.py:27);
         prev = (c1, den2) of the last step taken.  Returns (entries, a_values) with a_values[k] the
-        scale factor after entry k."""
-        ents = (_lib.Pm3dSchedEntry * max_steps)()
-        avals = []
-        a = a_start
-        c1p, denp = prev
-        k = 0
-        while k < max_steps and a < a_end:
-            da = min(da_frac * a, a_end - a)
-            sc = step_scalars(a, da, self.smooth, self.dx, False)
-            e = ents[k]
-            e.c1p, e.denp, e.c1, e.den1, e.da, e.den_d = c1p, denp, sc.c1, sc.den1, sc.da, sc.den_d
-            e.den2, e.s, e.sm2 = sc.den2, sc.s_new, sc.sm2
-            c1p, denp = sc.c1, sc.den2
-            a = a + da
-            avals.append(a)
-            k += 1
-        return ents, avals
+        scale factor after entry k.  Tables depend only on their arguments, the smoothing and dx, so
+        boxes started alike share one (the table is read, never written, by set_schedule)."""
+        ents, avals = _schedule_table(float(a_start), float(da_frac), float(a_end), int(max_steps),
+                                      float(prev[0]), float(prev[1]), float(self.smooth), float(self.dx))
+        return ents, list(avals)
 
     def set_schedule(self, entries, n: int):
         check(self._L.pm3d_set_schedule(self._h, entries, n), self._h)

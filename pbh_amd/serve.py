@@ -126,6 +126,7 @@ class CollapseServer:
         self.engine_bytes = {}  # N -> bytes of one engine
         self.held = 0           # bytes held by slots and engines
         self.active = []        # (Request, Member) in the captured graph
+        self.graph = []         # their Members in the order of the last capture
         self.joining = []       # admitted while replays ran; captured at the next re-capture
         self.queue = deque()
         self.completed = []
@@ -266,19 +267,23 @@ class CollapseServer:
         return n
 
     def _collect(self, now: float) -> bool:
-        """Status of every replayed member; completes the finished ones.  True if any finished."""
+        """Status of every replayed member (one device pass, Batch.status); completes those whose
+        schedule ended (at the first crossing or its last entry).  True if any finished."""
+        st = self.batch.status(len(self.graph))
+        where = {id(m): k for k, m in enumerate(self.graph)}
         still = []
         for req, m in self.active:
+            done, nsteps, _ = st[where[id(m)]]
+            if done < nsteps:
+                still.append((req, m))
+                continue
             done, hist = m.pm.schedule_status()
             m.jhist = [m.jhist[0]] + list(hist)
             m.a_vals = [m.a_vals[0]] + m._sched_a[:done]
             neg = np.nonzero(np.asarray(m.jhist) <= 0.0)[0]
-            if len(neg) or done >= len(m._sched_a):
-                _finish(m, int(neg[0]) if len(neg) else None)
-                m.pm.schedule_end(m.a_vals[-1])
-                self._complete(req, m, now)
-            else:
-                still.append((req, m))
+            _finish(m, int(neg[0]) if len(neg) else None)
+            m.pm.schedule_end(m.a_vals[-1])
+            self._complete(req, m, now)
         changed = len(still) != len(self.active)
         self.active = still
         return changed
@@ -320,7 +325,8 @@ class CollapseServer:
                     time.sleep(max(0.0, min(pend[0][0] - clock(), 0.01)))
                 continue
             if dirty:
-                self.batch.set([m.pm for _, m in self.active])
+                self.graph = [m for _, m in self.active]
+                self.batch.set([m.pm for m in self.graph])
                 self.graph_builds += 1
                 dirty = False
             self.max_active = max(self.max_active, len(self.active))

@@ -93,6 +93,15 @@ class StubBatch:
     def synchronize(self):
         pass
 
+    def status(self, n):
+        """(done, nsteps, jmin) per member; a table ends (nsteps = done) at the crossing."""
+        assert n == len(self.members)
+        out = []
+        for pm in self.members:
+            crossed = pm.done > 0 and pm._j(1 + pm.done) <= 0.0
+            out.append((pm.done, pm.done if crossed else pm.sched, pm._j(1 + pm.done) if pm.done else 1e300))
+        return out
+
 
 def make_server(budget, sizes, fail_mu=None, bad_mu=None, cross=None, **kw):
     """sizes: N -> bytes of a slot (engines take the same); cross: mu -> steps to crossing."""
