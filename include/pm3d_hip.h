@@ -244,13 +244,16 @@ int pm3d_schedule_step(void* handle);                             /* one schedul
 int pm3d_schedule_status(void* handle, int* done, double* jhist); /* steps done, jmin history (done entries) */
 int pm3d_schedule_end(void* handle, int* done);                   /* hand the state back to pm3d_step */
 
-/* A batch replays one captured hipGraph holding one scheduled step of every member as
- * independent branches (members of any mesh size and precision). */
+/* A batch replays one captured hipGraph holding `steps` scheduled steps (1 unless set with
+ * pm3d_batch_set_steps) of every member as independent branches (members of any mesh size and
+ * precision); a member's table ends on the device at its first shell crossing, after which its
+ * kernels do nothing. */
 int pm3d_batch_create(void** batch);
 int pm3d_batch_destroy(void* batch);
 const char* pm3d_batch_last_error(void* batch);
+int pm3d_batch_set_steps(void* batch, int steps);                 /* steps per graph launch, 1..64; next capture */
 int pm3d_batch_set(void* batch, void* const* handles, int count); /* (re)capture for these members */
-int pm3d_batch_run(void* batch, int nsteps);                      /* replay nsteps times, asynchronous */
+int pm3d_batch_run(void* batch, int nsteps);                      /* advance nsteps steps (whole launches, rounded up), asynchronous */
 int pm3d_batch_synchronize(void* batch);
 
 /* Schedule status of every member of the batch, in the order of pm3d_batch_set, read in one

@@ -94,6 +94,7 @@ SIGNATURES = {
     "pm3d_batch_last_error": (c_char_p, [c_void_p]),
     "pm3d_batch_set": (c_int, [c_void_p, POINTER(c_void_p), c_int]),
     "pm3d_batch_run": (c_int, [c_void_p, c_int]),
+    "pm3d_batch_set_steps": (c_int, [c_void_p, c_int]),
     "pm3d_batch_synchronize": (c_int, [c_void_p]),
     "pm3d_batch_status": (c_int, [c_void_p, POINTER(Pm3dMemberStatus), c_int]),
 }

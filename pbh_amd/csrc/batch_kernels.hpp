@@ -107,7 +107,7 @@ __global__ __launch_bounds__(1024) void k_deposit_col_boxes(const MxBox* __restr
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox* __restrict__ boxes, int seg) {
     const MxBox b = load_box(boxes, blockIdx.z);
     if (box_done(b)) return;
-    deposit_far_seg_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, seg, b.N);
+    deposit_far_seg_body<SrcMx, false>(box_src(b), b.mesh, b.far, b.farcnt, seg, b.N, nullptr);
 }
 
 __global__ __launch_bounds__(kThreads) void k_deposit_far_bal_boxes(const MxBox* __restrict__ boxes, int ncol, int seg) {

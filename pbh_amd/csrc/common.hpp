@@ -279,7 +279,13 @@ struct FarRows {
     int* ovl;             // overflow entries, flat
     unsigned int* ovcnt;  // their count (appends past ovcap are counted but not stored)
     unsigned int ovcap;   // entries ovl holds
+    // the device schedule of a scheduled (graph-replayed) deposit, nullptr otherwise: once its table
+    // has ended, the deposit kernels leave the mesh, the lists and their counts as they are, so a
+    // replay past the end recomputes the same force meshes (the gather-kick-drift, which fills the
+    // lists, does nothing then either)
+    const DevSched* sch;
 };
+__device__ __forceinline__ bool sched_ended(const DevSched* sch) { return sch && sch->counter >= sch->nsteps; }
 
 // Appends particle i to its row's list for the lanes with `far` set.  The lanes of a wave are
 // consecutive lattice sites, whose far particles come in runs of lanes with one row: each run takes

@@ -204,6 +204,7 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
     __shared__ unsigned long long acch[WIDE ? NCELL : 1];
     __shared__ unsigned int nfar;
     __shared__ unsigned int rpre[NR + 1];
+    if (sched_ended(fr.sch)) return;  // a replay past the end of the table: the mesh stays as it is
     const bool lists = fr.cnt != nullptr;
     const int col = (int)xcd_map(blockIdx.x, gridDim.x, xm);  // column id, also this column's far segment
     constexpr int NBY = N / BY;
@@ -381,7 +382,9 @@ constexpr int kFarSub = 4;
 template <class Src, bool WIDE>
 __device__ __forceinline__ void deposit_far_seg_body(Src src, unsigned long long* __restrict__ mesh,
                                                      const int* __restrict__ far,
-                                                     const unsigned int* __restrict__ farcnt, int seg, int N) {
+                                                     const unsigned int* __restrict__ farcnt, int seg, int N,
+                                                     const DevSched* __restrict__ sch) {
+    if (sched_ended(sch)) return;  // the columns left the mesh as it was: nothing to add
     const unsigned int cnt = farcnt[blockIdx.x];
     const int* list = far + (size_t)blockIdx.x * seg;
     for (unsigned int t = blockIdx.y * kThreads + threadIdx.x; t < cnt; t += gridDim.y * kThreads)
@@ -391,8 +394,9 @@ __device__ __forceinline__ void deposit_far_seg_body(Src src, unsigned long long
 template <class Src, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg(Src src, unsigned long long* __restrict__ mesh,
                                                               const int* __restrict__ far,
-                                                              const unsigned int* __restrict__ farcnt, int seg, int N) {
-    deposit_far_seg_body<Src, WIDE>(src, mesh, far, farcnt, seg, N);
+                                                              const unsigned int* __restrict__ farcnt, int seg, int N,
+                                                              const DevSched* __restrict__ sch) {
+    deposit_far_seg_body<Src, WIDE>(src, mesh, far, farcnt, seg, N, sch);
 }
 
 // Far particles, load-balanced: the far particles of a collapsing peak crowd into the few columns
@@ -411,6 +415,7 @@ __device__ __forceinline__ void deposit_far_bal_body(Src src, unsigned long long
                                                      const int* __restrict__ far,
                                                      const unsigned int* __restrict__ farcnt, int ncol, int seg,
                                                      int N, unsigned blk, unsigned nblk, FarRows fr, int lists) {
+    if (sched_ended(fr.sch)) return;
     if (lists && fr.cnt) {
         // the columns took the far particles from the row lists (and recorded none themselves): only
         // the lists' overflow and the clearing of the row counts are left

@@ -311,6 +311,17 @@ __global__ __launch_bounds__(kThreads) void k_sched_advance(DevSched* __restrict
     sched_advance_body(sch, bmin, nb, jhist);
 }
 
+// Minimum of nb per-block minima into *out (one block), so a host read of min det(dx/dq) copies 8 bytes.
+__global__ __launch_bounds__(kThreads) void k_min_reduce(const double* __restrict__ bmin, int nb, double* __restrict__ out) {
+    long long o = ordered_bits(1e300);
+    for (int t = threadIdx.x; t < nb; t += kThreads) {
+        const long long v = ordered_bits(bmin[t]);
+        o = (t == (int)threadIdx.x || v < o) ? v : o;
+    }
+    const double m = block_min(from_ordered_bits(o));
+    if (threadIdx.x == 0) *out = m;
+}
+
 // min det(dx/dq) over the Lagrangian lattice from cell + offset: the forward differences in cells
 // (exact integer cell differences, minimum image, plus the offset differences) form J*c with c the
 // cells per lattice site, so det(J) = det(J*c) * inv_c3, inv_c3 = 1/c^3 (one multiply instead of nine

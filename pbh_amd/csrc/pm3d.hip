@@ -125,7 +125,10 @@ struct Handle {
     DevStep* tab = nullptr;
     double* jhist = nullptr;
     int tab_cap = 0;
-    std::vector<DevStep> htab;
+    DevStep* htab = nullptr;    // pinned host copy of the schedule table (htab_cap entries)
+    int htab_cap = 0;
+    DevSched* hsched = nullptr; // pinned host copy of the counter the last pm3d_set_schedule uploaded
+    bool sched_upload = false;  // its copies may still read htab / hsched
     bool use_sched = false;  // set while a batch graph is being captured
     bool state_set = false;
     bool acc_valid = false;   // acc holds the acceleration at the current positions
@@ -237,9 +240,10 @@ int check_launch(Handle* h, const char* what) {
 // Far-particle row lists of the handle (FarRows), or none (cnt == nullptr) when its deposit does
 // not use columns or uses the per-column far pass (N = 512).
 FarRows far_rows(const Handle* h) {
-    if (!h->rowcnt) return FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u};
+    const DevSched* sch = h->use_sched ? h->sched : nullptr;
+    if (!h->rowcnt) return FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u, sch};
     return FarRows{h->rowcnt, (int*)(h->mixed ? (void*)h->accf : (void*)h->acc), h->row_cap, h->far,
-                   (unsigned int*)(h->dscal + 2), (unsigned int)h->n};
+                   (unsigned int*)(h->dscal + 2), (unsigned int)h->n, sch};
 }
 // the far-particle overflow count, cleared by the forward transform after the deposit consumed it
 unsigned int* ovf_clear(const Handle* h) { return h->rowcnt ? (unsigned int*)(h->dscal + 2) : nullptr; }
@@ -620,7 +624,7 @@ void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh, bool l
     {
         Timed t(h, KC_DEPOSIT, true);
         t.launch(k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024>, S::NBLK, 1024, src, mesh, h->far, h->farcnt,
-                 lists ? fr : FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u}, column_map<N, S::BY>(S::NBLK));
+                 lists ? fr : FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u, fr.sch}, column_map<N, S::BY>(S::NBLK));
     }
     Timed t(h, KC_DEPOSIT_FAR, true);
     if constexpr (S::NBLK <= kFarColsMax) {
@@ -630,7 +634,7 @@ void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh, bool l
         // a column lists at most its own BX*BY*N sites: kFarSub workgroups per column bound the work
         // of the fullest one
         t.launch(k_deposit_far_seg<Src, WIDE>, dim3(S::NBLK, kFarSub), kThreads, src, mesh, h->far, h->farcnt,
-                 S::BX * S::BY * N, N);
+                 S::BX * S::BY * N, N, fr.sch);
     }
 }
 
@@ -1001,6 +1005,8 @@ void free_all(Handle* h) {
     for (auto e : h->ev_pool) hipEventDestroy(e);
     for (auto e : h->live.ev)
         if (e) hipEventDestroy(e);
+    if (h->htab) hipHostFree(h->htab);
+    if (h->hsched) hipHostFree(h->hsched);
     if (h->st) hipStreamDestroy(h->st);
 }
 
@@ -1173,6 +1179,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_status(const BatchRef* __res
 
 struct Batch {
     hipStream_t st = nullptr;
+    int steps_per_graph = 1;              // scheduled steps one graph launch advances (pm3d_batch_set_steps)
     BatchRef* refs = nullptr;             // device, one per member
     pm3d_member_status* stat = nullptr;   // pinned host memory the status pass writes
     int ref_cap = 0;
@@ -1398,6 +1405,7 @@ __global__ __launch_bounds__(kThreads) void k_probe_read(const probe_v4* __restr
 extern "C" {
 
 constexpr int kMaxNp = 812;
+constexpr int kSchedHostCap = 512;  // schedule steps the pinned host table of a new handle holds
 
 int pm3d_create(const pm3d_cfg* cfg, void** handle) {
     if (!cfg || !handle) {
@@ -1477,13 +1485,22 @@ int pm3d_create(const pm3d_cfg* cfg, void** handle) {
             (rc = dalloc(h, &h->accf, n3)))
             return bail(rc);
     }
-    h->bmax_len = std::max<size_t>((size_t)N * N, (size_t)grid_for(h->n));
+    // per-block maxima / minima, plus one slot for the reduced min det(dx/dq) (pm3d_min_jacobian)
+    h->bmax_len = std::max<size_t>((size_t)N * N, (size_t)grid_for(h->n)) + 1;
     if ((rc = dalloc(h, &h->mesh, (h->wide ? 2 : 1) * h->M)) || (rc = dalloc_bytes(h, &h->spec, specN * 2 * rb)) ||
         (rc = dalloc_bytes(h, &h->gspec, 3 * specN * 2 * rb)) || (rc = dalloc_bytes(h, &h->grad, 3 * h->M * rb)) ||
         (rc = dalloc_bytes(h, &h->tw, (size_t)N * 2 * rb)) || (rc = dalloc(h, &h->dscal, 8 + kRing)) ||
         (rc = dalloc(h, &h->bmax, h->bmax_len)) || (rc = dalloc(h, &h->far, (size_t)h->n)) ||
         (rc = dalloc(h, &h->farcnt, (size_t)N * N)))  // far count per column
         return bail(rc);
+    // pinned host copies of a schedule (pm3d_set_schedule), allocated here so that setting a
+    // schedule allocates nothing for tables of up to kSchedHostCap steps
+    if (hipHostMalloc((void**)&h->hsched, sizeof(DevSched), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h->htab, kSchedHostCap * sizeof(DevStep), hipHostMallocDefault) != hipSuccess) {
+        h->err = "pinned host schedule buffers: allocation failed";
+        return bail(PM3D_ENOMEM);
+    }
+    h->htab_cap = kSchedHostCap;
     // twiddles exp(-2 pi i k/N) from long-double cos/sin
     std::vector<double> tw(2 * N);
     std::vector<float> twf(2 * N);
@@ -1888,10 +1905,12 @@ int pm3d_min_jacobian(void* handle, double* jmin) {
     Handle* h = (Handle*)handle;
     if (!h->state_set) return fail(h, PM3D_ESTATE, "state not set");
     const unsigned g = launch_min_jacobian(h);
-    std::vector<double> hb(g);
-    HCHK(h, hipMemcpyAsync(hb.data(), h->bmax, g * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    double* slot = h->bmax + (h->bmax_len - 1);
+    k_min_reduce<<<1, kThreads, 0, h->st>>>(h->bmax, (int)g, slot);
+    double v = 0.0;
+    HCHK(h, hipMemcpyAsync(&v, slot, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HCHK(h, hipStreamSynchronize(h->st));
-    *jmin = *std::min_element(hb.begin(), hb.end());
+    *jmin = v;
     return check_launch(h, "min_jacobian");
 }
 
@@ -2214,6 +2233,10 @@ int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps) {
     Handle* h = (Handle*)handle;
     if (!h->state_set || !h->grad_valid || !h->kick_pending)
         return fail(h, PM3D_ESTATE, "a schedule continues a stepped state: run one pm3d_step first");
+    if (h->sched_upload) {  // the previous table's copies read htab / hsched
+        HCHK(h, hipStreamSynchronize(h->st));
+        h->sched_upload = false;
+    }
     if (nsteps > h->tab_cap) {
         HCHK(h, hipStreamSynchronize(h->st));
         if (h->tab) HCHK(h, hipFree(h->tab));
@@ -2225,11 +2248,19 @@ int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps) {
         if ((rc = dalloc(h, &h->tab, (size_t)nsteps)) || (rc = dalloc(h, &h->jhist, (size_t)nsteps))) return rc;
         h->tab_cap = nsteps;
     }
+    if (nsteps > h->htab_cap) {  // a table longer than the pinned copy made at creation
+        if (h->htab) HCHK(h, hipHostFree(h->htab));
+        h->htab_cap = 0;
+        if (hipHostMalloc((void**)&h->htab, (size_t)nsteps * sizeof(DevStep), hipHostMallocDefault) != hipSuccess) {
+            h->htab = nullptr;
+            return fail(h, PM3D_ENOMEM, "pinned host schedule table allocation failed");
+        }
+        h->htab_cap = nsteps;
+    }
     if (!h->sched) {
         int rc;
         if ((rc = dalloc(h, &h->sched, 1))) return rc;
     }
-    h->htab.resize(nsteps);
     for (int k = 0; k < nsteps; ++k) {
         const pm3d_sched_entry& e = table[k];
         DevStep& d = h->htab[k];
@@ -2241,10 +2272,13 @@ int pm3d_set_schedule(void* handle, const pm3d_sched_entry* table, int nsteps) {
         d.sd = (float)(e.da / (e.den_d * h->dx));
         d.pad = 0.f;
     }
-    DevSched hs{0, nsteps, 0, 0};
-    HCHK(h, hipMemcpyAsync(h->tab, h->htab.data(), nsteps * sizeof(DevStep), hipMemcpyHostToDevice, h->st));
-    HCHK(h, hipMemcpyAsync(h->sched, &hs, sizeof(hs), hipMemcpyHostToDevice, h->st));
-    HCHK(h, hipStreamSynchronize(h->st));
+    // the copies read the handle's host table and counter, which stay untouched until the next
+    // pm3d_set_schedule waits for them; no synchronisation here, so the tables of many members
+    // upload while their first steps run
+    *h->hsched = DevSched{0, nsteps, 0, 0};
+    HCHK(h, hipMemcpyAsync(h->tab, h->htab, nsteps * sizeof(DevStep), hipMemcpyHostToDevice, h->st));
+    HCHK(h, hipMemcpyAsync(h->sched, h->hsched, sizeof(DevSched), hipMemcpyHostToDevice, h->st));
+    h->sched_upload = true;
     return 0;
 }
 
@@ -2252,10 +2286,16 @@ int pm3d_schedule_status(void* handle, int* done, double* jhist) {
     if (!handle || !done) return PM3D_EINVAL;
     Handle* h = (Handle*)handle;
     if (!h->sched) return fail(h, PM3D_ESTATE, "no schedule");
+    // on the handle's stream: ordered after its schedule upload and its own scheduled steps (a
+    // batch's replays are synchronised by pm3d_batch_status / pm3d_batch_synchronize)
     DevSched hs;
-    HCHK(h, hipMemcpy(&hs, h->sched, sizeof(hs), hipMemcpyDeviceToHost));
+    HCHK(h, hipMemcpyAsync(&hs, h->sched, sizeof(hs), hipMemcpyDeviceToHost, h->st));
+    HCHK(h, hipStreamSynchronize(h->st));
     *done = hs.counter;
-    if (jhist && hs.counter > 0) HCHK(h, hipMemcpy(jhist, h->jhist, hs.counter * sizeof(double), hipMemcpyDeviceToHost));
+    if (jhist && hs.counter > 0) {
+        HCHK(h, hipMemcpyAsync(jhist, h->jhist, hs.counter * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HCHK(h, hipStreamSynchronize(h->st));
+    }
     return 0;
 }
 
@@ -2420,13 +2460,19 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
         if (branches) {
             hipStream_t cs = b->cls_st[c];
             e = hipStreamWaitEvent(cs, b->fork, 0);
-            if (e == hipSuccess) launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), cs);
-            if (e == hipSuccess) e = hipGetLastError();
+            // a class runs its steps of the launch one after the other without waiting for the
+            // other classes between them
+            for (int k = 0; k < b->steps_per_graph && e == hipSuccess; ++k) {
+                launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), cs);
+                e = hipGetLastError();
+            }
             if (e == hipSuccess) e = hipEventRecord(b->cls_join[c], cs);
             if (e == hipSuccess) e = hipStreamWaitEvent(b->st, b->cls_join[c], 0);
         } else {
-            launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), b->st);
-            e = hipGetLastError();
+            for (int k = 0; k < b->steps_per_graph && e == hipSuccess; ++k) {
+                launch_boxes(h0, cls_dev[c].second, (int)classes[h0->N].size(), b->st);
+                e = hipGetLastError();
+            }
         }
     }
     for (int i = 0; i < count && e == hipSuccess && rc == 0; ++i) {
@@ -2435,7 +2481,7 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
         const bool timing = h->timing;
         h->timing = false;
         e = hipStreamWaitEvent(h->st, b->fork, 0);
-        if (e == hipSuccess) rc = launch_sched_step(h);
+        for (int k = 0; k < b->steps_per_graph && e == hipSuccess && rc == 0; ++k) rc = launch_sched_step(h);
         if (e == hipSuccess && rc == 0) e = hipEventRecord(b->joins[i], h->st);
         if (e == hipSuccess && rc == 0) e = hipStreamWaitEvent(b->st, b->joins[i], 0);
         h->timing = timing;
@@ -2457,11 +2503,18 @@ int pm3d_batch_set(void* batch, void* const* handles, int count) {
     return 0;
 }
 
+int pm3d_batch_set_steps(void* batch, int steps) {
+    if (!batch || steps < 1 || steps > 64) return PM3D_EINVAL;
+    ((Batch*)batch)->steps_per_graph = steps;
+    return 0;
+}
+
 int pm3d_batch_run(void* batch, int nsteps) {
     if (!batch || nsteps < 0) return PM3D_EINVAL;
     Batch* b = (Batch*)batch;
     if (!b->exec) return 0;
-    for (int k = 0; k < nsteps; ++k) {
+    const int launches = (nsteps + b->steps_per_graph - 1) / b->steps_per_graph;
+    for (int k = 0; k < launches; ++k) {
         hipError_t e = hipGraphLaunch(b->exec, b->st);
         if (e != hipSuccess) {
             b->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);

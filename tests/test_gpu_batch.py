@@ -109,13 +109,15 @@ def test_batch_state_equals_alone(precision):
         assert np.array_equal(u.pm.x, v.pm.x)
 
 
-def test_batched_collapse_equals_plain():
-    """First shell crossing of boxes of mesh 32-128 (mixed): batched == plain, bitwise t_c."""
-    from pbh_amd.batch import run_batched, run_plain
-    say("batched collapse == plain")
+@pytest.mark.parametrize("steps_per_graph", [1, 8])
+def test_batched_collapse_equals_plain(steps_per_graph):
+    """First shell crossing of boxes of mesh 32-128 (mixed): batched == plain, bitwise t_c, with one
+    step or eight steps per graph launch."""
+    from pbh_amd.batch import Batch, run_batched, run_plain
+    say("batched collapse == plain, steps per graph", steps_per_graph)
     seeds = [100, 104, 108, 112, 116, 120]
     rp = run_plain(_members(seeds, "mixed"))
-    rb = run_batched(_members(seeds, "mixed"), poll=8)
+    rb = run_batched(_members(seeds, "mixed"), poll=8, batch=Batch(steps_per_graph=steps_per_graph))
     for x, y in zip(rp, rb):
         assert x.crossed and y.crossed
         assert x.steps == y.steps
