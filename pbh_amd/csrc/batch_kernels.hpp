@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx_boxes(const MxBox*
 
 __global__ __launch_bounds__(kThreads) void k_sched_advance_boxes(const MxBox* __restrict__ boxes, int nb) {
     const MxBox b = load_box(boxes, blockIdx.y);
-    sched_advance_body(b.sched, b.bmax, nb, b.jhist);
+    sched_advance_body(b.sched, b.bmax, nb, b.jhist, b.dscal + 1);
 }
 
 }  // namespace pm3d

@@ -290,9 +290,11 @@ __global__ __launch_bounds__(kThreads) void k_f32_soa_to_f64(const float* __rest
 
 // Last kernel of a scheduled step: reduce the Jacobian kernel's per-block minima into
 // jhist[counter] and advance the schedule.  A minimum <= 0 (the first shell crossing) ends the
-// table there (nsteps = counter), so the member's later steps do nothing.
+// table there (nsteps = counter), so the member's later steps do nothing; so does a non-zero count
+// of non-finite particles (`bad`, the handle's sticky counter), so a broken member stops at once.
 __device__ __forceinline__ void sched_advance_body(DevSched* __restrict__ sch, const double* __restrict__ bmin,
-                                                   int nb, double* __restrict__ jhist) {
+                                                   int nb, double* __restrict__ jhist,
+                                                   const unsigned long long* __restrict__ bad) {
     double m = 1e300;
     for (int t = threadIdx.x; t < nb; t += kThreads) m = fmin(m, bmin[t]);
     m = block_min(m);
@@ -301,14 +303,15 @@ __device__ __forceinline__ void sched_advance_body(DevSched* __restrict__ sch, c
         if (k < sch->nsteps) {
             jhist[k] = m;
             sch->counter = k + 1;
-            if (m <= 0.0) sch->nsteps = k + 1;
+            if (m <= 0.0 || *bad != 0ull) sch->nsteps = k + 1;
         }
     }
 }
 
 __global__ __launch_bounds__(kThreads) void k_sched_advance(DevSched* __restrict__ sch, const double* __restrict__ bmin,
-                                                            int nb, double* __restrict__ jhist) {
-    sched_advance_body(sch, bmin, nb, jhist);
+                                                            int nb, double* __restrict__ jhist,
+                                                            const unsigned long long* __restrict__ bad) {
+    sched_advance_body(sch, bmin, nb, jhist, bad);
 }
 
 // Minimum of nb per-block minima into *out (one block), so a host read of min det(dx/dq) copies 8 bytes.

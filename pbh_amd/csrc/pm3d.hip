@@ -1156,7 +1156,7 @@ int launch_sched_step(Handle* h) {
     h->use_sched = false;
     if (rc) return rc;
     const unsigned g = launch_min_jacobian(h);
-    k_sched_advance<<<1, kThreads, 0, h->st>>>(h->sched, h->bmax, (int)g, h->jhist);
+    k_sched_advance<<<1, kThreads, 0, h->st>>>(h->sched, h->bmax, (int)g, h->jhist, h->dscal + 1);
     return check_launch(h, "scheduled step");
 }
 
