@@ -141,8 +141,9 @@ struct SrcMx {
 // corner pair whose weight is 0 in every lane of the wave (a ring row's far plane, a zero fraction,
 // an idle lane) is skipped, and within a pair only lanes with a non-zero weight issue the atomic.
 // Each corner weight (wx*wy)*wz is rounded to the fixed-point grid on its own.  Called by every lane
-// of the wave (`valid` false for idle lanes).
-template <class Src, int BX, int BY, int NZ, bool WIDE>
+// of the wave (`valid` false for idle lanes).  WIDE && SINGLE: the 128-bit cell is summed in one
+// 64-bit accumulator (q52 whole: no cell of the column can reach 2^64, see deposit_col_body).
+template <class Src, int BX, int BY, int NZ, bool WIDE, bool SINGLE>
 __device__ __forceinline__ void col_add(const Src& src, unsigned long long* acc, unsigned long long* acch, bool valid,
                                         int lx, int ly, int lz, const typename Src::W f[3]) {
     using W = typename Src::W;
@@ -166,7 +167,9 @@ __device__ __forceinline__ void col_add(const Src& src, unsigned long long* acc,
             // only lanes with a contribution add: idle lanes (far particles at their sites, corners
             // outside the column) would otherwise pile zero adds onto shared LDS addresses
             if (w == W(0)) continue;
-            if constexpr (WIDE) {
+            if constexpr (WIDE && SINGLE) {
+                atomicAdd(&acc[a + cz], src.quant(w));
+            } else if constexpr (WIDE) {
                 unsigned l32, h32;
                 src.quant2(w, l32, h32);
                 atomicAdd(&acc[a + cz], (unsigned long long)l32);
@@ -198,10 +201,17 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
     constexpr bool ROWS = N % 64 == 0;
     constexpr int RY = BY + 1, NR = (BX + 1) * RY;  // base-cell rows whose corners reach the column
     static_assert(NR < NT, "one thread per row list");
-    // WIDE: the 128-bit cell is accumulated as two 64-bit sums of the low and high 32-bit halves of
-    // each contribution (no return value, no carry test per add) and recombined exactly at the end
+    // WIDE: a cell c takes near particles whose base cell is c or c-1 on every axis, i.e. sites c-2 ..
+    // c+1 (base = site + {-1, 0, 1}): at most 64, and one contribution per entry of the four row lists
+    // at and below it; each contribution is at most 2^52 units.  Rows hold at most 6N entries (the
+    // acceleration buffer they alias), so for 64 + 24N < 4096, N <= 167, no cell reaches 2^64 and the
+    // cell's 64-bit sum is exact in any order: SINGLE, one accumulator per cell and one LDS atomic
+    // per corner.  Larger meshes accumulate the 128-bit cell as two 64-bit sums of the low and high
+    // 32-bit halves of each contribution (no return value, no carry test per add), recombined
+    // exactly at the end.
+    constexpr bool SINGLE = WIDE && 64 + 24 * N < 4096;
     __shared__ unsigned long long acc[NCELL];
-    __shared__ unsigned long long acch[WIDE ? NCELL : 1];
+    __shared__ unsigned long long acch[WIDE && !SINGLE ? NCELL : 1];
     __shared__ unsigned int nfar;
     __shared__ unsigned int rpre[NR + 1];
     if (sched_ended(fr.sch)) return;  // a replay past the end of the table: the mesh stays as it is
@@ -217,7 +227,7 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
     }
     for (int t = threadIdx.x; t < NCELL; t += NT) {
         acc[t] = 0ull;
-        if (WIDE) acch[t] = 0ull;
+        if (WIDE && !SINGLE) acch[t] = 0ull;
     }
     if (threadIdx.x == 0) nfar = 0u;
     __syncthreads();
@@ -271,7 +281,7 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
                 if (app) myfar[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)((sx * N + sy) * N + sz);
             }
             // column-local planes of the base cell: ex-1+ob in [-2, BX]
-            col_add<Src, BX, BY, NZ, WIDE>(src, acc, acch, is_near, ex - 1 + ob[0], ey - 1 + ob[1], lo[2], f);
+            col_add<Src, BX, BY, NZ, WIDE, SINGLE>(src, acc, acch, is_near, ex - 1 + ob[0], ey - 1 + ob[1], lo[2], f);
         }
     }
     if (lists) {
@@ -326,7 +336,7 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
                     src.cell_frac(raw[u], 1, lxy, f[1]);
                     src.cell_frac(raw[u], 2, lz, f[2]);
                 }
-                col_add<Src, BX, BY, NZ, WIDE>(src, acc, acch, valid[u], ra[u] / RY - 1, ra[u] % RY - 1, lz, f);
+                col_add<Src, BX, BY, NZ, WIDE, SINGLE>(src, acc, acch, valid[u], ra[u] / RY - 1, ra[u] % RY - 1, lz, f);
             }
         }
     }
@@ -335,7 +345,7 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
     if (threadIdx.x < BX * BY) {  // fold z = N into z = 0
         const int r = threadIdx.x * NZ;
         acc[r] += acc[r + N];
-        if (WIDE) acch[r] += acch[r + N];
+        if (WIDE && !SINGLE) acch[r] += acch[r + N];
     }
     __syncthreads();
     for (int t = threadIdx.x; t < BX * BY * N; t += NT) {
@@ -343,7 +353,9 @@ __device__ __forceinline__ void deposit_col_body(Src src, unsigned long long* __
         const int ry = rxy % BY, rx = rxy / BY;
         const unsigned cell = (unsigned)(((x0 + rx) * N + (y0 + ry)) * N + rz);
         const int l = rxy * NZ + rz;
-        if (WIDE) {
+        if (WIDE && SINGLE) {
+            *(ulonglong2*)(mesh + 2 * (size_t)cell) = make_ulonglong2(acc[l], 0ull);
+        } else if (WIDE) {
             const unsigned long long L = acc[l], H = acch[l];
             const unsigned long long lo = L + (H << 32);
             *(ulonglong2*)(mesh + 2 * (size_t)cell) = make_ulonglong2(lo, (H >> 32) + (lo < L ? 1ull : 0ull));

@@ -265,6 +265,29 @@ def test_tiled_deposit_equals_atomic(PM3D, fb):
     assert np.array_equal(d_step, rho / 1.0 - 1.0)
 
 
+@pytest.mark.parametrize("N,squeeze,shape", [(32, 0.5, "0"), (32, 0.5, "2"), (256, 0.4, "auto")])
+def test_crowded_columns_keep_the_128bit_sums(PM3D, N, squeeze, shape, monkeypatch):
+    """FP64 column deposit: a cell takes near particles from at most 64 sites and the entries of the
+    four row lists at and below it (at most 6N each).  Meshes with 64 + 24N < 4096 sum each 128-bit
+    cell in one 64-bit accumulator, larger ones as low and high 32-bit halves.  A lattice squeezed
+    around the box centre puts every particle in the central rows: at 32^3 (one accumulator) the
+    rows are full, at 256^3 (halves) they overflow into the global pass.  The step's density equals
+    the reference-shaped atomic deposit of the same positions bit for bit."""
+    monkeypatch.setenv("PM3D_DEPOSIT_SHAPE", shape)
+    say("crowded columns", N, squeeze, shape)
+    L = 1.0
+    q = (np.arange(N) + 0.5) * L / N
+    X = np.stack(np.meshgrid(q, q, q, indexing="ij"), -1).reshape(-1, 3)
+    X = 0.5 * L + squeeze * (X - 0.5 * L)
+    pm = PM3D(N, L, N)
+    pm.set_state(X, np.zeros_like(X))
+    del X
+    a, dm = pm.step(0.01, 1e-6)  # its second deposit takes the far particles from the row lists
+    a, dm = pm.step(a, 1e-6)
+    assert np.array_equal(np.asarray(dm), pm.deposit(pm.x) / 1.0 - 1.0)
+    pm.close()
+
+
 def test_driver_rows_match_oracle_loop(tmp_path):
     """The drop-in driver (16^3 mesh, 8^3 particles, to t = 3 t_H) against the same loop on the oracle."""
     import json
