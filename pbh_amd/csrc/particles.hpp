@@ -309,4 +309,72 @@ __global__ __launch_bounds__(kThreads) void k_min_jacobian(const double* __restr
     if (threadIdx.x == 0) bmin[blockIdx.x] = m;
 }
 
+// v - L*rint(v/L), the minimum image of a difference of two positions in [0, L), without the
+// division where the quotient's rounding cannot matter: |v| < lo = 0.49 L gives rint = +-0 (the
+// sign of v), lo' = 0.51 L < |v| < hi = 1.49 L gives +-1; the bands in between and beyond take the
+// division.  L*(+-0) and L*(+-1) are exact, so the result is the expression's bit for bit.
+struct MinImage {
+    double L, lo, mid, hi;
+};
+__device__ __forceinline__ double min_image(double v, const MinImage& m) {
+    const double a = fabs(v);
+    double k;
+    if (a < m.lo) k = 0.0;
+    else if (a > m.mid && a < m.hi) k = 1.0;
+    else k = fabs(rint(v / m.L));
+    return v - m.L * copysign(k, v);
+}
+
+// min det(dx/dq) with four consecutive lattice sites of one z row per thread (Np % 4 == 0): the
+// positions of the four sites and of their +x and +y neighbours are two 16-byte loads per axis and
+// array, the +z neighbour of a site is the next site of the four (only the fourth's is loaded on its
+// own).  Entries: min_image, then the quotient by dq as qdiv with the host's reciprocal rdq (the
+// IEEE quotient on the CPU checks of tests/test_div_exact.py).  Block b holds the minimum of sites
+// [1024 b, 1024 b + 1024).
+__global__ __launch_bounds__(kThreads) void k_min_jacobian4(const double* __restrict__ x, unsigned n, MinImage mi,
+                                                            double dq, double rdq, SiteDiv sdv,
+                                                            double* __restrict__ bmin) {
+    const unsigned np = sdv.Np;
+    const unsigned i = 4u * (blockIdx.x * kThreads + threadIdx.x);
+    long long ob = ordered_bits(1e300);
+    if (i < n) {
+        int st[3];
+        site_of(i, sdv, st);
+        const unsigned a = (unsigned)st[0], b = (unsigned)st[1], c = (unsigned)st[2];
+        const unsigned ixn = i + ((a + 1 == np ? 0u : a + 1) - a) * np * np;  // +x neighbour of site i
+        const unsigned iyn = i + ((b + 1 == np ? 0u : b + 1) - b) * np;       // +y neighbour
+        const unsigned izn = i - c + (c + 4 == np ? 0u : c + 4);              // +z neighbour of site i + 3
+        double X[3][4], XX[3][4], XY[3][4], xz[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double* xd = x + (size_t)d * n;
+            const double2 p0 = ld_off((const double2*)xd, i * 8u), p1 = ld_off((const double2*)xd, i * 8u + 16u);
+            const double2 q0 = ld_off((const double2*)xd, ixn * 8u), q1 = ld_off((const double2*)xd, ixn * 8u + 16u);
+            const double2 r0 = ld_off((const double2*)xd, iyn * 8u), r1 = ld_off((const double2*)xd, iyn * 8u + 16u);
+            X[d][0] = p0.x, X[d][1] = p0.y, X[d][2] = p1.x, X[d][3] = p1.y;
+            XX[d][0] = q0.x, XX[d][1] = q0.y, XX[d][2] = q1.x, XX[d][3] = q1.y;
+            XY[d][0] = r0.x, XY[d][1] = r0.y, XY[d][2] = r1.x, XY[d][3] = r1.y;
+            xz[d] = ld_off(xd, izn * 8u);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double J[3][3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double xi = X[d][k];
+                const double nb[3] = {XX[d][k], XY[d][k], k < 3 ? X[d][k + 1 < 4 ? k + 1 : 3] : xz[d]};
+#pragma unroll
+                for (int e = 0; e < 3; ++e) J[e][d] = qdiv(min_image(nb[e] - xi, mi), dq, rdq);
+            }
+            const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                               J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            const long long o = ordered_bits(det);
+            ob = (k == 0 || o < ob) ? o : ob;
+        }
+    }
+    const double m = block_min(from_ordered_bits(ob));
+    if (threadIdx.x == 0) bmin[blockIdx.x] = m;
+}
+
 }  // namespace pm3d

@@ -140,3 +140,72 @@ def test_magic_division_of_lattice_sites(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     tot, bad = map(int, out.stdout.split())
     assert tot > 0 and bad == 0
+
+
+JAC_SRC = r"""
+#include <stdio.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+static uint64_t s = 0xD1B54A32D192ED03ULL;
+static double urand(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return (s >> 11) * 0x1p-53; }
+/* the four-site FP64 Jacobian's entry (csrc/particles.hpp min_image + qdiv) against the one-site
+   kernel's (v - L*rint(v/L)) / dq, compared as bit patterns (the sign of zero included) */
+static double min_image(double v, double L, double lo, double mid, double hi) {
+  const double a = fabs(v);
+  double k;
+  if (a < lo) k = 0.0;
+  else if (a > mid && a < hi) k = 1.0;
+  else k = fabs(rint(v / L));
+  return v - L * copysign(k, v);
+}
+int main(void) {
+  const double Ls[] = {1.0, 19.595917942265423, 3.0, 2.5, 7.3, 1e-3, 123.456};
+  const int Ns[] = {8, 16, 32, 48, 64, 96, 128, 256, 512, 812};
+  long bad = 0, tot = 0;
+  for (int a = 0; a < 7; ++a) for (int b = 0; b < 10; ++b) {
+    const double L = Ls[a], dq = L / Ns[b], rdq = 1.0 / dq;
+    const double lo = 0.49 * L, mid = 0.51 * L, hi = 1.49 * L;
+    for (long k = 0; k < 200000; ++k) {
+      double x1 = urand() * L, x2;
+      switch (k % 6) {
+        case 0: x2 = urand() * L; break;                                    /* any pair */
+        case 1: x2 = fmod(x1 + dq * (1.0 + 0.3 * (urand() - 0.5)), L); break; /* lattice neighbours */
+        case 2: x2 = x1 + dq * 1e-9 * (urand() - 0.5); break;                /* nearly coincident */
+        case 3: x1 = urand() * dq; x2 = L - urand() * dq; break;             /* across the boundary */
+        case 4: x2 = x1; break;                                              /* zero difference */
+        default: {                                                           /* the band edges */
+          const double e[6] = {lo, mid, hi, 0.5 * L, L, 1.5 * L};
+          double v = e[k % 6 == 5 ? (k / 6) % 6 : 0] * (k & 64 ? -1.0 : 1.0);
+          v = (k & 128) ? nextafter(v, 0) : ((k & 256) ? nextafter(v, 1e300 * v) : v);
+          x1 = 0.0; x2 = v;
+        }
+      }
+      const double v = x2 - x1;
+      const double ref = (v - L * rint(v / L)) / dq;
+      const double w = min_image(v, L, lo, mid, hi);
+      const double q0 = w * rdq;
+      const double got = w == 0.0 ? q0 : fma(fma(-q0, dq, w), rdq, q0);
+      if (memcmp(&ref, &got, sizeof(double)) != 0) ++bad;
+      ++tot;
+    }
+  }
+  printf("%ld %ld\n", tot, bad);
+  return bad != 0;
+}
+"""
+
+
+def test_jacobian_entry_without_divisions_is_bitwise(tmp_path):
+    """The FP64 Jacobian's division-free minimum image and reciprocal quotient equal the plain
+    expression bit for bit (sign of zero included) over positions of every workload box and mesh."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "jac.c"
+    src.write_text(JAC_SRC)
+    exe = tmp_path / "jac"
+    subprocess.run([gcc, "-O2", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    tot, bad = map(int, out.stdout.split())
+    assert tot == 7 * 10 * 200000 and bad == 0
