@@ -368,22 +368,28 @@ __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx(MxPos P, int Np, i
     min_jacobian_mx_body(P, Np, N, inv_c3, bmin);
 }
 
-// The same minimum with four consecutive lattice sites (one z row, Np % 4 == 0) per thread: the
-// cell and offset words of the four sites and of their +x and +y neighbours are one 16-byte load
-// per array (18 loads for four sites instead of 96 4-byte loads), the +z neighbour of a site is
-// the next site of the four, and only the fourth one's is loaded on its own.  Each site's
-// determinant is the arithmetic of min_jacobian_mx_body; the per-thread minimum is taken on the
-// ordered bits, like block_min.  Block b holds the minimum of sites [1024 b, 1024 b + 1024).
+// The determinant of one site's forward differences (the arithmetic of min_jacobian_mx_body).
+// POW2 (N = 2^k, k >= 1, sh = 32 - k): the minimum image of a cell difference dc in (-N, N) is the
+// sign-extended low k bits of dc, which lie in [-N/2, N/2), except dc = +N/2, which the compare form
+// keeps as +N/2: one bit-field extract, one compare and one select instead of two compares, two adds
+// and two selects, and the same integer for every dc in (-N, N) (cells of the state are in [0, N)).
+// Compiled, the four-site loop takes 412 VALU instructions instead of 522.
+template <bool POW2>
 __device__ __forceinline__ double det_mx(const int ci[3], const float fi[3], const int cn[3][3], const float fn[3][3],
-                                         int N, double inv_c3) {
+                                         int N, int sh, double inv_c3) {
     double J[3][3];
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             int dc = cn[e][d] - ci[d];
-            if (dc > N / 2) dc -= N;
-            if (dc < -N / 2) dc += N;
+            if constexpr (POW2) {
+                const int w = __builtin_amdgcn_sbfe(dc, 0u, 32u - (unsigned)sh);  // v_bfe_i32
+                dc = dc == N / 2 ? dc : w;
+            } else {
+                if (dc > N / 2) dc -= N;
+                if (dc < -N / 2) dc += N;
+            }
             J[e][d] = (double)dc + ((double)fn[e][d] - (double)fi[d]);
         }
     }
@@ -391,8 +397,16 @@ __device__ __forceinline__ double det_mx(const int ci[3], const float fi[3], con
             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) * inv_c3;
 }
 
-__device__ __forceinline__ void min_jacobian_mx4_body(MxPos P, int N, double inv_c3, SiteDiv sdv,
-                                                      double* __restrict__ bmin) {
+// The same minimum with four consecutive lattice sites (one z row, Np % 4 == 0) per thread: the
+// cell and offset words of the four sites and of their +x and +y neighbours are one 16-byte load
+// per array (18 loads for four sites instead of 96 4-byte loads), the +z neighbour of a site is
+// the next site of the four, and only the fourth one's is loaded on its own.  Each site's
+// determinant is the arithmetic of min_jacobian_mx_body; the per-thread minimum is taken on the
+// ordered bits, like block_min.  Block b holds the minimum of sites [1024 b, 1024 b + 1024).
+template <bool POW2>
+__device__ __forceinline__ void min_jacobian_mx4_sites(MxPos P, int N, double inv_c3, SiteDiv sdv,
+                                                       double* __restrict__ bmin) {
+    const int sh = POW2 ? 32 - __builtin_ctz((unsigned)N) : 0;
     const unsigned n = (unsigned)P.n, np = sdv.Np;
     const unsigned i = 4u * (blockIdx.x * kThreads + threadIdx.x);
     long long ob = ordered_bits(1e300);
@@ -439,12 +453,19 @@ __device__ __forceinline__ void min_jacobian_mx4_body(MxPos P, int N, double inv
                 cn[2][d] = k < 3 ? cv[k + 1 < 4 ? k + 1 : 3] : cz[d];
                 fn[2][d] = k < 3 ? fv[k + 1 < 4 ? k + 1 : 3] : fz[d];
             }
-            const long long o = ordered_bits(det_mx(ci, fi, cn, fn, N, inv_c3));
+            const long long o = ordered_bits(det_mx<POW2>(ci, fi, cn, fn, N, sh, inv_c3));
             ob = (k == 0 || o < ob) ? o : ob;
         }
     }
     const double m = block_min(from_ordered_bits(ob));
     if (threadIdx.x == 0) bmin[blockIdx.x] = m;
+}
+
+// N is uniform over the launch (over a box's blocks in the batch): one branch picks the form.
+__device__ __forceinline__ void min_jacobian_mx4_body(MxPos P, int N, double inv_c3, SiteDiv sdv,
+                                                      double* __restrict__ bmin) {
+    if (N >= 2 && (N & (N - 1)) == 0) min_jacobian_mx4_sites<true>(P, N, inv_c3, sdv, bmin);
+    else min_jacobian_mx4_sites<false>(P, N, inv_c3, sdv, bmin);
 }
 
 __global__ __launch_bounds__(kThreads) void k_min_jacobian_mx4(MxPos P, int N, double inv_c3, SiteDiv sdv,

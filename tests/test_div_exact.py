@@ -209,3 +209,42 @@ def test_jacobian_entry_without_divisions_is_bitwise(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     tot, bad = map(int, out.stdout.split())
     assert tot == 7 * 10 * 200000 and bad == 0
+
+
+WRAP_SRC = r"""
+#include <stdio.h>
+/* the mixed Jacobian's minimum image of a cell difference (csrc/particles_mixed.hpp det_mx): the
+   power-of-two form against the compare form, every dc in (-N, N) */
+int main(void) {
+  long bad = 0, tot = 0;
+  for (int k = 1; k <= 10; ++k) {
+    const int N = 1 << k, sh = 32 - k;
+    for (int dc = -N + 1; dc < N; ++dc) {
+      int r = dc;
+      if (r > N / 2) r -= N;
+      if (r < -N / 2) r += N;
+      const int w = (int)((unsigned)dc << sh) >> sh;
+      const int p = dc == N / 2 ? dc : w;
+      if (p != r) ++bad;
+      ++tot;
+    }
+  }
+  printf("%ld %ld\n", tot, bad);
+  return bad != 0;
+}
+"""
+
+
+def test_mixed_minimum_image_power_of_two_form(tmp_path):
+    """Sign extension of the low log2 N bits with the +N/2 tie kept equals the two-compare minimum
+    image for every cell difference of meshes 2..1024."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "wrap.c"
+    src.write_text(WRAP_SRC)
+    exe = tmp_path / "wrap"
+    subprocess.run([gcc, "-O2", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    tot, bad = map(int, out.stdout.split())
+    assert tot == sum(2 * (1 << k) - 1 for k in range(1, 11)) and bad == 0
