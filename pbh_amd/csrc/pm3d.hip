@@ -17,6 +17,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <iterator>
 #include <array>
 #include <cstdlib>
 #include <map>
@@ -72,6 +73,11 @@ const char* kKernelNames =
 constexpr int kLivePeriod = 64;  // step deposits between the starts of two trials
 constexpr int kLiveReps = 2;     // deposits per candidate in a trial (its faster one counts)
 constexpr int kLiveSlots = 4 * kLiveReps;
+// A candidate slower than the trial's winner by more than this factor sits out the next
+// kLiveRestTrials trials: its deposits are the trial's cost (FP64 128^3 at the crossing: 8x8
+// columns 44 us against 32 for 4x4, about 1.5 us per step over a window that holds a trial).
+constexpr float kLiveRestMargin = 1.2f;
+constexpr int kLiveRestTrials = 3;
 struct LiveShape {
     unsigned cand = 0;       // bit v: shape variant v is a candidate (0 or one bit: no trials)
     long long deposits = 0;  // step deposits made
@@ -80,6 +86,7 @@ struct LiveShape {
     int nplan = 0, pos = 0;     // trial length (0: none running), deposits of it launched
     int unread = 0;             // samples of the trial not yet read
     float best[4] = {};         // fastest sample per variant in the running trial (ms)
+    int rest[4] = {};           // trials a candidate sits out after losing by more than kLiveRestMargin
     hipEvent_t ev[2 * kLiveSlots] = {};  // event pair of each sample, in plan order
     int trials = 0, changes = 0;         // finished trials, shape changes
 };
@@ -758,6 +765,9 @@ void live_read(Handle* h) {
         int best = h->depv;
         for (int v = 0; v < 4; ++v)
             if ((t.cand >> v & 1u) && t.best[v] < 0.98f * t.best[h->depv] && t.best[v] < t.best[best]) best = v;
+        for (int v = 0; v < 4; ++v)  // timed this trial (best < 1e30) and far behind: rest
+            if ((t.cand >> v & 1u) && v != best && t.best[v] < 1e30f && t.best[v] > kLiveRestMargin * t.best[best])
+                t.rest[v] = kLiveRestTrials;
         t.changes += best != h->depv;
         h->depv = best;
         t.nplan = t.pos = 0;
@@ -776,8 +786,12 @@ int live_deposit(Handle* h) {
         t.next = t.deposits + kLivePeriod;
         for (int v = 0; v < 4; ++v) {
             t.best[v] = 1e30f;
-            if ((t.cand >> v & 1u) && v != h->depv)
-                for (int r = 0; r < kLiveReps; ++r) t.plan[t.nplan++] = v;
+            if (!(t.cand >> v & 1u) || v == h->depv) continue;
+            if (t.rest[v] > 0) {
+                --t.rest[v];
+                continue;
+            }
+            for (int r = 0; r < kLiveReps; ++r) t.plan[t.nplan++] = v;
         }
         for (int r = 0; r < kLiveReps; ++r) t.plan[t.nplan++] = h->depv;
         t.pos = t.unread = 0;
@@ -1638,7 +1652,8 @@ int pm3d_set_state(void* handle, const double* x, const double* p, int64_t n) {
     }
     HCHK(h, hipStreamSynchronize(h->st));
     h->state_set = true;
-    h->live.next = h->live.deposits;  // a trial at the new state's first step
+    h->live.next = h->live.deposits;  // a trial at the new state's first step, every candidate in it
+    std::fill(std::begin(h->live.rest), std::end(h->live.rest), 0);
     return check_launch(h, "set_state");
 }
 
@@ -2209,7 +2224,8 @@ int pm3d_make_peak_ic(void* handle, const pm3d_peak_ic* ic) {
     k_lattice<<<grid_for(h->n), kThreads, 0, h->st>>>((const double*)h->grad, h->N, h->Np, h->L, ic->coef, h->x, h->p);
     HCHK(h, hipStreamSynchronize(h->st));
     h->state_set = true;
-    h->live.next = h->live.deposits;  // a trial at the new state's first step
+    h->live.next = h->live.deposits;  // a trial at the new state's first step, every candidate in it
+    std::fill(std::begin(h->live.rest), std::end(h->live.rest), 0);
     h->acc_valid = false;
     h->grad_valid = false;
     h->kick_pending = false;
@@ -2232,7 +2248,8 @@ int pm3d_copy_state(void* dst_handle, void* src_handle) {
     }
     HCHK(d, hipStreamSynchronize(d->st));
     d->state_set = true;
-    d->live.next = d->live.deposits;  // a trial at the new state's first step
+    d->live.next = d->live.deposits;  // a trial at the new state's first step, every candidate in it
+    std::fill(std::begin(d->live.rest), std::end(d->live.rest), 0);
     d->acc_valid = false;
     d->grad_valid = false;
     d->kick_pending = false;
