@@ -14,7 +14,7 @@ timed steps are the ones that END at the first shell crossing (the collapse phas
 crowd into few cells): value = particle-steps/s = n*K/wall.  A second pass over the same K steps
 times every step kernel with the start/stop events its dispatch stamps (`kernels`, `roofline` for
 the kernel with the largest share).  `cpu_baseline` = the numpy FP64 oracle on the same workload,
-3 steps, single-threaded (1 core).
+20 steps from the initial conditions (about 11 s of CPU work), single-threaded (1 core).
 
 Extra keys of the same line (skipped with --no-extras):
   early_window     K steps after the warm-up (from a_i), the same kernel table;
@@ -314,7 +314,7 @@ def main():
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--da-frac", type=float, default=0.0125)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--serve-rate", type=float, default=8.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-n", type=int, default=200, help="serving: number of requests")
