@@ -160,23 +160,28 @@ def test_mixed_vs_fp64_at_256():
     assert rel(pmx.p, pm64.p) <= 1e-5
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("N,Np", [(32, 32), (32, 28), (32, 30), (48, 48)])
-def test_mixed_min_jacobian_matches_oracle_on_state(N, Np):
-    """min det(dx/dq) of the mixed state against the oracle's on the positions read back from it
+def test_mixed_min_jacobian_matches_oracle_on_state(N, Np, precision):
+    """min det(dx/dq) of the state against the oracle's on the positions read back from it
     (Np % 4 == 0: four lattice sites per thread; Np = 28 also covers rows of 7 quads across
-    waves; otherwise one site per thread)."""
+    waves; otherwise one site per thread).  FP64: the four-site kernel's division-free entries and
+    the one-site kernel's IEEE quotients give the oracle's minimum bit for bit."""
     from pbh_amd.pm3d import PM3D
-    say(f"mixed min_jacobian N={N} Np={Np}")
+    say(f"{precision} min_jacobian N={N} Np={Np}")
     L = 1.0
     q = (np.arange(Np) + 0.5) * L / Np
     X = np.stack(np.meshgrid(q, q, q, indexing="ij"), -1).reshape(-1, 3)
     rng = np.random.default_rng(7)
     amp = 0.45 * L / Np  # displacements up to about half a lattice spacing: folds somewhere
     X = (X + amp * np.sin(2 * np.pi * X[:, [1, 2, 0]] / L) + amp * 0.3 * rng.standard_normal(X.shape)) % L
-    pm = PM3D(N, L, Np, precision="mixed")
+    pm = PM3D(N, L, Np, precision=precision)
     pm.set_state(X, np.zeros_like(X))
     got = pm.min_jacobian()
     ref = O.min_jacobian(pm.x, Np, L)
-    say(f"  gpu {got:.12e} oracle {ref:.12e}")
-    assert got == pytest.approx(ref, rel=1e-9, abs=1e-12)
+    say(f"  gpu {got:.17e} oracle {ref:.17e}")
+    if precision == "fp64":
+        assert got == ref
+    else:
+        assert got == pytest.approx(ref, rel=1e-9, abs=1e-12)
     pm.close()
