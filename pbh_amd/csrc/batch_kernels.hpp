@@ -98,10 +98,10 @@ __global__ __launch_bounds__(kThreads) void k_gather_kick_drift_mx_boxes(const M
 }
 
 template <int BX, int BY, int N>
-__global__ __launch_bounds__(1024) void k_deposit_col_boxes(const MxBox* __restrict__ boxes, XMap xm) {
+__global__ __launch_bounds__(SrcMx::kColThreads) void k_deposit_col_boxes(const MxBox* __restrict__ boxes, XMap xm) {
     const MxBox b = load_box(boxes, blockIdx.y);
     if (box_done(b)) return;
-    deposit_col_body<SrcMx, BX, BY, N, false, 1024>(box_src(b), b.mesh, b.far, b.farcnt, b.fr, xm);
+    deposit_col_body<SrcMx, BX, BY, N, false, SrcMx::kColThreads>(box_src(b), b.mesh, b.far, b.farcnt, b.fr, xm);
 }
 
 __global__ __launch_bounds__(kThreads) void k_deposit_far_seg_boxes(const MxBox* __restrict__ boxes, int seg) {

@@ -37,6 +37,7 @@ struct SrcF64T {
     double dx, inv_dx;
     int N;
     using W = double;
+    static constexpr int kColThreads = 1024;
     struct Raw {
         double v[3];
     };
@@ -95,6 +96,7 @@ struct SrcMx {
     const float* f2;
     int N;
     using W = float;
+    static constexpr int kColThreads = 512;
     struct Raw {
         int c[3];
         float f[3];
@@ -180,6 +182,13 @@ __device__ __forceinline__ void col_add(const Src& src, unsigned long long* acc,
         }
     }
 }
+
+// Threads per column workgroup.  Mixed state (4-byte cells and offsets, 8-byte mesh cells): 512, so
+// up to four 4x4 columns share a CU (peak256 at its crossing: deposit 213 -> 190 us and the tuner's
+// choice 8x8 -> 4x4 columns; early steps 153 -> 141 us).  FP64: 1024 (512 made the 4x4 columns at
+// the 128^3 crossing 31.5 -> 33.2 us and the 8x8 ones 43 -> 60 us).
+template <class Src>
+constexpr int col_threads() { return Src::kColThreads; }
 
 // Column deposit.  LDS holds the column as BX*BY rows of N+1 cells: entry N of a row stands for
 // z = 0 (the upper z corner of a particle whose base cell is N-1) and is folded into entry 0 before

@@ -631,7 +631,8 @@ void deposit_columns(Handle* h, const Src& src, unsigned long long* mesh, bool l
     const FarRows fr = far_rows(h);
     {
         Timed t(h, KC_DEPOSIT, true);
-        t.launch(k_deposit_col<Src, S::BX, S::BY, N, WIDE, 1024>, S::NBLK, 1024, src, mesh, h->far, h->farcnt,
+        constexpr int NT = col_threads<Src>();
+        t.launch(k_deposit_col<Src, S::BX, S::BY, N, WIDE, NT>, S::NBLK, NT, src, mesh, h->far, h->farcnt,
                  lists ? fr : FarRows{nullptr, nullptr, 0, nullptr, nullptr, 0u, fr.sch}, column_map<N, S::BY>(S::NBLK));
     }
     Timed t(h, KC_DEPOSIT_FAR, true);
@@ -1266,7 +1267,7 @@ MxBox box_of(Handle* h) {
 template <int N, int V>
 void launch_boxes_deposit(Handle* h0, const MxBox* dev, int nb, hipStream_t st) {
     using S = ColShape<N, false, V>;
-    k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), 1024, 0, st>>>(dev, column_map<N, S::BY>(S::NBLK));
+    k_deposit_col_boxes<S::BX, S::BY, N><<<dim3(S::NBLK, nb), col_threads<SrcMx>(), 0, st>>>(dev, column_map<N, S::BY>(S::NBLK));
     if constexpr (S::NBLK <= kFarColsMax)
         k_deposit_far_bal_boxes<<<dim3(h0->rowcnt ? far_lists_grid(N) : far_bal_grid(h0->n), nb), kThreads, 0, st>>>(
             dev, S::NBLK, S::BX * S::BY * N);
