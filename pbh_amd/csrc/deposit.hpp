@@ -186,7 +186,8 @@ __device__ __forceinline__ void col_add(const Src& src, unsigned long long* acc,
 // Threads per column workgroup.  Mixed state (4-byte cells and offsets, 8-byte mesh cells): 512, so
 // up to four 4x4 columns share a CU (peak256 at its crossing: deposit 213 -> 190 us and the tuner's
 // choice 8x8 -> 4x4 columns; early steps 153 -> 141 us).  FP64: 1024 (512 made the 4x4 columns at
-// the 128^3 crossing 31.5 -> 33.2 us and the 8x8 ones 43 -> 60 us).
+// the 128^3 crossing 31.5 -> 33.2 us and the 8x8 ones 43 -> 60 us).  256 was slower for both
+// (FP64 4x4 at the crossing 44.5 us, mixed 214 us).
 template <class Src>
 constexpr int col_threads() { return Src::kColThreads; }
 
