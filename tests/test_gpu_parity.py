@@ -344,7 +344,8 @@ def test_fft_tile_variants(PM3D, N, prec, monkeypatch):
             assert np.array_equal(u, v), t
 
 
-@pytest.mark.parametrize("N,prec", [(128, "fp64"), (256, "mixed"), (96, "fp64"), (64, "mixed"), (16, "fp64")])
+@pytest.mark.parametrize("N,prec", [(128, "fp64"), (256, "mixed"), (96, "fp64"), (96, "mixed"), (64, "mixed"),
+                                    (16, "fp64")])
 def test_deposit_shape_variants(PM3D, N, prec, monkeypatch):
     """Every deposit column shape sums the same fixed-point corner weights, so a step's positions,
     momenta and density contrast agree bit for bit across the shapes that fit, and the automatic
