@@ -17,14 +17,14 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
-#include <iterator>
 #include <array>
-#include <cstdlib>
-#include <map>
-#include <mutex>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
