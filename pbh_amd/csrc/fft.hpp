@@ -13,9 +13,10 @@
 // kz = N/2 Nyquist plane lives in a separate [kx][ky] array `nyq`.  numpy's layout (N,N,N/2+1) is
 // assembled only on the host side of the test entry points.
 //
-// LDS layout of a pencil block: element j of pencil c at buf[j*PP + c], PP = lds_pad(P) (P+1, or P+2
-// for FP32 with P = 8); threads map pencil-fastest in the butterflies, so every LDS access of a wave
-// is a run of contiguous pencils.
+// LDS layout of a pencil block: element j of pencil c at buf[lds_at(j, c)], rows of P elements
+// (j*PP + c with PP = lds_pad(P) = P + 1, or rows of 8 in padded groups of 8 for FP32 with P = 8);
+// threads map pencil-fastest in the butterflies, so every LDS access of a wave is a run of
+// contiguous pencils.
 #pragma once
 #include "common.hpp"
 
@@ -121,24 +122,43 @@ __device__ __forceinline__ void dft_reg(cplx<R>* v) {
 }
 
 // Row stride of a P-pencil LDS tile buf[j*PP + c].  Threads take consecutive pencils, so a wave
-// reads 64/P rows of P elements.  P + 1 spreads the rows over the banks, except for 2-dword
-// elements (FP32 complex) with P = 8, where a stride of 2*(P+1) = 18 dwords starts the 8 rows of a
-// wave unevenly (about 2e7 bank-conflict cycles per 256^3 x-pass launch on MI355X); P + 2 starts
-// them on every fourth bank, so each bank serves 4 of the wave's 128 dwords.
+// reads 64/P rows of P elements.  P + 1 spreads the rows over the banks.  (FP32 complex with P = 8
+// takes the grouped layout of lds_at instead: a stride of 18 dwords started the 8 rows of a wave
+// unevenly, about 2e7 bank-conflict cycles per 256^3 x-pass launch on MI355X, and 20 dwords
+// still left the 64-bank conflicts of 1.6x the active LDS cycles measured at the crossing.)
 template <typename R>
-constexpr int lds_pad(int P) { return P + (sizeof(R) == 4 && P == 8 ? 2 : 1); }
+constexpr int lds_pad(int P) { return P + 1; }
+
+// Element (row, pencil c) of a P-pencil LDS tile, and the elements a tile of `rows` rows takes.
+// FP32 with P = 8: rows of 8 elements (16 dwords) in groups of 8 rows, each group followed by 8
+// unused elements (64 dwords; a group spans 144 dwords).  In a model of 64 four-byte banks, every
+// wave-wide access of the pencil passes (rows j .. j+7 of a wave, the radix-4 Stockham reads
+// j + r*M and writes (j/Ns)*Ns*4 + r*Ns + j%Ns) then reaches each bank the minimum number of times,
+// where a stride of P + 2 needs 1.5x that; the padding is additive, so two accesses of one thread
+// keep a constant distance (LDS instruction pairing).  Other tiles keep lds_pad.
+template <typename R, int P>
+constexpr bool lds_grouped() { return sizeof(R) == 4 && P == 8; }
+template <typename R, int P>
+__device__ __forceinline__ int lds_at(int row, int c) {
+    if constexpr (lds_grouped<R, P>()) return row * 8 + (row >> 3) * 8 + c;
+    else return row * lds_pad<R>(P) + c;
+}
+template <typename R, int P>
+constexpr int lds_size(int rows) {
+    return lds_grouped<R, P>() ? rows * 8 + ((rows + 7) >> 3) * 8 : rows * lds_pad<R>(P);
+}
 
 // Stockham pass of radix RADIX on P pencils (see lds_fft).
 template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
 __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, int Ns, const cplx<R>* __restrict__ tw) {
-    constexpr int PP = lds_pad<R>(P), M = NC / RADIX;
+    constexpr int M = NC / RADIX;
     const int s = TWS * (NC / (RADIX * Ns));
     for (int t = threadIdx.x; t < P * M; t += kThreads) {
         const int c = t % P, j = t / P;
         const int k = j % Ns;
         cplx<R> v[RADIX];
 #pragma unroll
-        for (int r = 0; r < RADIX; ++r) v[r] = src[(j + r * M) * PP + c];
+        for (int r = 0; r < RADIX; ++r) v[r] = src[lds_at<R, P>(j + r * M, c)];
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < RADIX; ++r) {
@@ -150,7 +170,7 @@ __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, 
         dft_reg<R, RADIX, INV>(v);
         const int dd = (j / Ns) * Ns * RADIX + k;
 #pragma unroll
-        for (int r = 0; r < RADIX; ++r) dst[(dd + r * Ns) * PP + c] = v[r];
+        for (int r = 0; r < RADIX; ++r) dst[lds_at<R, P>(dd + r * Ns, c)] = v[r];
     }
     __syncthreads();
 }
@@ -159,7 +179,7 @@ __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, 
 // the DFT in registers, and writes back after a barrier.
 template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
 __device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, int Ns, const cplx<R>* __restrict__ tw) {
-    constexpr int PP = lds_pad<R>(P), M = NC / RADIX, T = P * M, IT = (T + kThreads - 1) / kThreads;
+    constexpr int M = NC / RADIX, T = P * M, IT = (T + kThreads - 1) / kThreads;
     const int s = TWS * (NC / (RADIX * Ns));
     cplx<R> v[IT][RADIX];
 #pragma unroll
@@ -168,7 +188,7 @@ __device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, int Ns, const cpl
         if (T % kThreads == 0 || t < T) {
             const int c = t % P, j = t / P, k = j % Ns;
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) v[it][r] = buf[(j + r * M) * PP + c];
+            for (int r = 0; r < RADIX; ++r) v[it][r] = buf[lds_at<R, P>(j + r * M, c)];
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < RADIX; ++r) {
@@ -188,7 +208,7 @@ __device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, int Ns, const cpl
             const int c = t % P, j = t / P, k = j % Ns;
             const int dd = (j / Ns) * Ns * RADIX + k;
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) buf[(dd + r * Ns) * PP + c] = v[it][r];
+            for (int r = 0; r < RADIX; ++r) buf[lds_at<R, P>(dd + r * Ns, c)] = v[it][r];
         }
     }
     __syncthreads();
@@ -232,8 +252,8 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
                                                     double* __restrict__ bmax, double qscale, double nmean,
                                                     unsigned int* __restrict__ clear) {
     if (clear && blockIdx.x == 0 && threadIdx.x == 0) *clear = 0u;
-    constexpr int NC = N / 2, P = Plan<R, N, V>::PZ, PP = lds_pad<R>(P);
-    __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
+    constexpr int NC = N / 2, P = Plan<R, N, V>::PZ;
+    __shared__ cplx<R> A[lds_size<R, P>(NC)], B[Plan<R, N, V>::INPLACE ? 1 : lds_size<R, P>(NC)];
     const size_t row0 = (size_t)blockIdx.x * P;
     double lmax = 0.0;
     const double inv_nmean = 1.0 / nmean;  // rho*(1/n_mean) - 1: exact for a power-of-two n_mean
@@ -251,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
             v0 = ((const R*)in)[g];
             v1 = ((const R*)in)[g + 1];
         }
-        A[m * PP + c] = cplx<R>{v0, v1};
+        A[lds_at<R, P>(m, c)] = cplx<R>{v0, v1};
     }
     if constexpr (FIXED) {
         const double m = block_max_nonneg(lmax);
@@ -261,8 +281,8 @@ __global__ __launch_bounds__(kThreads) void k_r2c_z(const void* __restrict__ in,
     const cplx<R>* Z = lds_fft<R, NC, P, false, 2, radix_l2<R, PK_R2C>(), Plan<R, N, V>::INPLACE>(A, B, tw);
     for (int t = threadIdx.x; t < P * NC; t += kThreads) {
         const int c = t / NC, k = t - c * NC;
-        const cplx<R> zk = Z[k * PP + c];
-        const cplx<R> zm = Z[(k == 0 ? 0 : NC - k) * PP + c];
+        const cplx<R> zk = Z[lds_at<R, P>(k, c)];
+        const cplx<R> zm = Z[lds_at<R, P>(k == 0 ? 0 : NC - k, c)];
         // even part E = (Z_k + conj Z_{-k})/2, odd part O = (Z_k - conj Z_{-k})/(2i)
         const R er = (zk.re + zm.re) * (R)0.5, ei = (zk.im - zm.im) * (R)0.5;
         const R orr = (zk.im + zm.im) * (R)0.5, oi = (zm.re - zk.re) * (R)0.5;
@@ -281,10 +301,10 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
                                                     size_t cs, size_t cn, const cplx<R>* __restrict__ tw,
                                                     const double* __restrict__ bmax, int nbmax,
                                                     unsigned long long* __restrict__ slot, XMap xm) {
-    constexpr int P = Plan<R, N, V>::PY, PP = lds_pad<R>(P), NH = N / 2, NBM = N * (NH / P);
+    constexpr int P = Plan<R, N, V>::PY, NH = N / 2, NBM = N * (NH / P);
     constexpr int ML2 = radix_l2<R, INV ? PK_YINV : PK_YFWD>();
     constexpr bool IP = Plan<R, N, V>::INPLACE;
-    __shared__ cplx<R> A[N * PP], B[IP ? 1 : N * PP];
+    __shared__ cplx<R> A[lds_size<R, P>(N)], B[IP ? 1 : lds_size<R, P>(N)];
     if (nbmax > 0 && blockIdx.x == gridDim.x - 1 && blockIdx.y == 0) {
         double m = 0.0;
         for (int t = threadIdx.x; t < nbmax; t += kThreads) m = fmax(m, bmax[t]);
@@ -300,25 +320,25 @@ __global__ __launch_bounds__(kThreads) void k_c2c_y(cplx<R>* __restrict__ spec, 
         cplx<R>* base = spec + (size_t)x * N * NH + col0;
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;
-            A[j * PP + c] = base[(size_t)j * NH + c];
+            A[lds_at<R, P>(j, c)] = base[(size_t)j * NH + c];
         }
         __syncthreads();
         const cplx<R>* res = lds_fft<R, N, P, INV, 1, ML2, IP>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;
-            base[(size_t)j * NH + c] = res[j * PP + c];
+            base[(size_t)j * NH + c] = res[lds_at<R, P>(j, c)];
         }
     } else {
         cplx<R>* base = nyq + (size_t)(b - NBM) * P * N;  // pencil c = row x0+c, contiguous in y
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int j = t % N, c = t / N;
-            A[j * PP + c] = base[(size_t)c * N + j];
+            A[lds_at<R, P>(j, c)] = base[(size_t)c * N + j];
         }
         __syncthreads();
         const cplx<R>* res = lds_fft<R, N, P, INV, 1, ML2, IP>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int j = t % N, c = t / N;
-            base[(size_t)c * N + j] = res[j * PP + c];
+            base[(size_t)c * N + j] = res[lds_at<R, P>(j, c)];
         }
     }
 }
@@ -357,11 +377,11 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
         sm2 = tab[k].sm2;
         smooth_on = sm2 > 0.0 ? 1 : 0;
     }
-    constexpr int P = Plan<R, N, V>::PX, PP = lds_pad<R>(P), NH = N / 2, NBM = N * (NH / P);
+    constexpr int P = Plan<R, N, V>::PX, NH = N / 2, NBM = N * (NH / P);
     constexpr int EPT = (N * P + kThreads - 1) / kThreads;
     constexpr int ML2 = radix_l2<R, PK_X>();
     constexpr bool IP = Plan<R, N, V>::INPLACE;
-    __shared__ cplx<R> A[N * PP], B[IP ? 1 : N * PP];
+    __shared__ cplx<R> A[lds_size<R, P>(N)], B[IP ? 1 : lds_size<R, P>(N)];
     // the twiddle table staged in LDS for the pencil FFTs (FP64 128^3: 22.1 -> 21.1 us); first read
     // after the barrier that ends the load phase
     __shared__ cplx<R> TW[N];
@@ -383,7 +403,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
     cplx<R>* src = main ? spec : nyq;
     for (int t = threadIdx.x; t < N * P; t += kThreads) {
         const int c = t % P, j = t / P;
-        A[j * PP + c] = src[off(j, c)];
+        A[lds_at<R, P>(j, c)] = src[off(j, c)];
     }
     __syncthreads();
     if constexpr (MODE == 3) {
@@ -395,7 +415,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
         for (int e = 0; e < EPT; ++e) {
             const int t = threadIdx.x + e * kThreads;
             dkr[e] = cplx<R>{(R)0, (R)0};
-            if (t < N * P) dkr[e] = res[(t / P) * PP + t % P];
+            if (t < N * P) dkr[e] = res[lds_at<R, P>(t / P, t % P)];
         }
         __syncthreads();
         for (int comp = 0; comp < 3; ++comp) {
@@ -413,7 +433,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
                     const double kc = comp == 0 ? kx : comp == 1 ? ky : kz;
                     double pr = (-(kc * (double)dkr[e].im)) * rk, pi = (kc * (double)dkr[e].re) * rk;
                     if (dc) { pr = 0.0; pi = 0.0; }
-                    A[j * PP + c] = cplx<R>{(R)pr, (R)pi};
+                    A[lds_at<R, P>(j, c)] = cplx<R>{(R)pr, (R)pi};
                 }
             }
             __syncthreads();
@@ -421,7 +441,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
             cplx<R>* dst = main ? gspec + comp * gcs : gnyq + comp * gcn;
             for (int t = threadIdx.x; t < N * P; t += kThreads) {
                 const int c = t % P, j = t / P;
-                dst[off(j, c)] = r2[j * PP + c];
+                dst[off(j, c)] = r2[lds_at<R, P>(j, c)];
             }
             __syncthreads();
         }
@@ -430,7 +450,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
         const cplx<R>* res = lds_fft<R, N, P, MODE == 1, 1, ML2, IP>(A, B, tw);
         for (int t = threadIdx.x; t < N * P; t += kThreads) {
             const int c = t % P, j = t / P;
-            src[off(j, c)] = res[j * PP + c];
+            src[off(j, c)] = res[lds_at<R, P>(j, c)];
         }
         return;
     } else {
@@ -448,7 +468,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
                 double k2 = (kx * kx + ky * ky) + kz * kz;
                 if (dc) k2 = 1.0;
                 const double rk = 1.0 / k2;
-                const cplx<R> z = res[j * PP + c];
+                const cplx<R> z = res[lds_at<R, P>(j, c)];
                 double pr = (s * (double)z.re) * rk, pi = (s * (double)z.im) * rk;
                 if (smooth_on) {
                     const double f = exp((-0.5 * k2) * sm2);
@@ -471,10 +491,10 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
                 if (t < N * P) {
                     const int c = t % P, j = t / P;
                     if (pass == 0) {
-                        A[j * PP + c] = ph[e];
+                        A[lds_at<R, P>(j, c)] = ph[e];
                     } else {
                         const double kx = kfull(j, N, kval);
-                        A[j * PP + c] = cplx<R>{(R)(kx * (double)ph[e].im), (R)(-(kx * (double)ph[e].re))};
+                        A[lds_at<R, P>(j, c)] = cplx<R>{(R)(kx * (double)ph[e].im), (R)(-(kx * (double)ph[e].re))};
                     }
                 }
             }
@@ -487,7 +507,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
                     const int c = t % P, j = t / P;
                     const int iy = main ? y : col0 + c, iz = main ? col0 + c : NH;
                     const double ky = kfull(iy, N, kval), kz = khalf(iz, kval);
-                    const cplx<R> z = r2[j * PP + c];
+                    const cplx<R> z = r2[lds_at<R, P>(j, c)];
                     d1[off(j, c)] = cplx<R>{(R)(ky * (double)z.im), (R)(-(ky * (double)z.re))};
                     d2[off(j, c)] = cplx<R>{(R)(kz * (double)z.im), (R)(-(kz * (double)z.re))};
                 }
@@ -495,7 +515,7 @@ __device__ __forceinline__ void x_pass_body(cplx<R>* __restrict__ spec, cplx<R>*
                 cplx<R>* d0 = main ? gspec : gnyq;
                 for (int t = threadIdx.x; t < N * P; t += kThreads) {
                     const int c = t % P, j = t / P;
-                    d0[off(j, c)] = r2[j * PP + c];
+                    d0[off(j, c)] = r2[lds_at<R, P>(j, c)];
                 }
             }
             __syncthreads();
@@ -524,8 +544,8 @@ template <typename R, int N, int NCOMP, int V>
 __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ gspec, const cplx<R>* __restrict__ gnyq,
                                                     size_t gcs, size_t gcn, R* __restrict__ out, size_t ocs,
                                                     const cplx<R>* __restrict__ tw, R scale) {
-    constexpr int NC = N / 2, P = Plan<R, N, V>::template pc<NCOMP>(), NP = NCOMP * P, PP = lds_pad<R>(NP);
-    __shared__ cplx<R> A[NC * PP], B[Plan<R, N, V>::INPLACE ? 1 : NC * PP];
+    constexpr int NC = N / 2, P = Plan<R, N, V>::template pc<NCOMP>(), NP = NCOMP * P;
+    __shared__ cplx<R> A[lds_size<R, NP>(NC)], B[Plan<R, N, V>::INPLACE ? 1 : lds_size<R, NP>(NC)];
     const size_t row0 = (size_t)blockIdx.x * P;
     for (int t = threadIdx.x; t < NP * NC; t += kThreads) {
         const int q = t / NC, k = t - q * NC;  // q = comp*P + c
@@ -538,7 +558,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ 
         const R dr = xk.re - xm.re, di = xk.im + xm.im;
         const cplx<R> w = tw[k];  // exp(-2 pi i k/N); the odd part takes exp(+2 pi i k/N)
         const R orr = dr * w.re + di * w.im, oi = di * w.re - dr * w.im;
-        A[k * PP + q] = cplx<R>{er - oi, ei + orr};
+        A[lds_at<R, NP>(k, q)] = cplx<R>{er - oi, ei + orr};
     }
     __syncthreads();
     const cplx<R>* z = lds_fft<R, NC, NP, true, 2, radix_l2<R, PK_C2R>(), Plan<R, N, V>::INPLACE>(A, B, tw);
@@ -547,7 +567,7 @@ __global__ __launch_bounds__(kThreads) void k_c2r_z(const cplx<R>* __restrict__ 
     for (int t = threadIdx.x; t < NP * N; t += kThreads) {
         const int q = t / N, e = t - q * N;  // q = comp*P + c
         const int comp = q / P, c = q - comp * P;
-        const cplx<R> v = z[(e >> 1) * PP + q];
+        const cplx<R> v = z[lds_at<R, NP>(e >> 1, q)];
         out[comp * ocs + (row0 + c) * N + e] = ((e & 1) ? v.im : v.re) * scale;
     }
 }
