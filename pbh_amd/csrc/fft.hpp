@@ -147,18 +147,36 @@ template <typename R, int P>
 constexpr int lds_size(int rows) {
     return lds_grouped<R, P>() ? rows * 8 + ((rows + 7) >> 3) * 8 : rows * lds_pad<R>(P);
 }
+// Element distance from (row, c) to (row + d, c) when (row % 8) + (d % 8) < 8, so that the rows
+// between cross d / 8 group ends: a constant for a constant d, which keeps the accesses of one
+// thread at immediate offsets of one address (the compiler cannot derive that from lds_at).
+template <typename R, int P>
+__device__ __forceinline__ constexpr int lds_adv(int d) {
+    return lds_grouped<R, P>() ? d * 8 + (d >> 3) * 8 : d * lds_pad<R>(P);
+}
+// A radix-RADIX pass reads rows j + r*M and writes rows dd + r*NS with dd = (j/NS)*NS*RADIX + k,
+// k < NS.  lds_adv holds for the reads when M % 8 == 0, and for the writes when NS % 8 == 0 or
+// NS and RADIX are powers of two (the NS*RADIX rows of dd's block then lie in whole groups, or in
+// one group, and dd % 8 + (r*NS) % 8 stays below 8).
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+template <typename R, int P, int M>
+constexpr bool lds_rd_adv() { return !lds_grouped<R, P>() || M % 8 == 0; }
+template <typename R, int P, int NS, int RADIX>
+constexpr bool lds_wr_adv() { return !lds_grouped<R, P>() || NS % 8 == 0 || (is_pow2(NS) && is_pow2(RADIX)); }
 
 // Stockham pass of radix RADIX on P pencils (see lds_fft).
-template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
-__device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, int Ns, const cplx<R>* __restrict__ tw) {
+template <typename R, int NC, int P, bool INV, int TWS, int RADIX, int Ns>
+__device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, const cplx<R>* __restrict__ tw) {
     constexpr int M = NC / RADIX;
-    const int s = TWS * (NC / (RADIX * Ns));
+    constexpr int s = TWS * (NC / (RADIX * Ns));
     for (int t = threadIdx.x; t < P * M; t += kThreads) {
         const int c = t % P, j = t / P;
         const int k = j % Ns;
         cplx<R> v[RADIX];
+        const int j0 = lds_at<R, P>(j, c);
 #pragma unroll
-        for (int r = 0; r < RADIX; ++r) v[r] = src[lds_at<R, P>(j + r * M, c)];
+        for (int r = 0; r < RADIX; ++r)
+            v[r] = src[lds_rd_adv<R, P, M>() ? j0 + lds_adv<R, P>(r * M) : lds_at<R, P>(j + r * M, c)];
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < RADIX; ++r) {
@@ -168,27 +186,29 @@ __device__ __forceinline__ void stockham_pass(const cplx<R>* src, cplx<R>* dst, 
             }
         }
         dft_reg<R, RADIX, INV>(v);
-        const int dd = (j / Ns) * Ns * RADIX + k;
+        const int dd = (j / Ns) * Ns * RADIX + k, d0 = lds_at<R, P>(dd, c);
 #pragma unroll
-        for (int r = 0; r < RADIX; ++r) dst[lds_at<R, P>(dd + r * Ns, c)] = v[r];
+        for (int r = 0; r < RADIX; ++r)
+            dst[lds_wr_adv<R, P, Ns, RADIX>() ? d0 + lds_adv<R, P>(r * Ns) : lds_at<R, P>(dd + r * Ns, c)] = v[r];
     }
     __syncthreads();
 }
 
 // The same pass on one buffer: every thread reads its RADIX-point inputs, applies the twiddles and
 // the DFT in registers, and writes back after a barrier.
-template <typename R, int NC, int P, bool INV, int TWS, int RADIX>
-__device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, int Ns, const cplx<R>* __restrict__ tw) {
+template <typename R, int NC, int P, bool INV, int TWS, int RADIX, int Ns>
+__device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, const cplx<R>* __restrict__ tw) {
     constexpr int M = NC / RADIX, T = P * M, IT = (T + kThreads - 1) / kThreads;
-    const int s = TWS * (NC / (RADIX * Ns));
+    constexpr int s = TWS * (NC / (RADIX * Ns));
     cplx<R> v[IT][RADIX];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int t = threadIdx.x + it * kThreads;
         if (T % kThreads == 0 || t < T) {
-            const int c = t % P, j = t / P, k = j % Ns;
+            const int c = t % P, j = t / P, k = j % Ns, j0 = lds_at<R, P>(j, c);
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) v[it][r] = buf[lds_at<R, P>(j + r * M, c)];
+            for (int r = 0; r < RADIX; ++r)
+                v[it][r] = buf[lds_rd_adv<R, P, M>() ? j0 + lds_adv<R, P>(r * M) : lds_at<R, P>(j + r * M, c)];
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < RADIX; ++r) {
@@ -206,9 +226,11 @@ __device__ __forceinline__ void stockham_pass_ip(cplx<R>* buf, int Ns, const cpl
         const int t = threadIdx.x + it * kThreads;
         if (T % kThreads == 0 || t < T) {
             const int c = t % P, j = t / P, k = j % Ns;
-            const int dd = (j / Ns) * Ns * RADIX + k;
+            const int dd = (j / Ns) * Ns * RADIX + k, d0 = lds_at<R, P>(dd, c);
 #pragma unroll
-            for (int r = 0; r < RADIX; ++r) buf[lds_at<R, P>(dd + r * Ns, c)] = v[it][r];
+            for (int r = 0; r < RADIX; ++r)
+                buf[lds_wr_adv<R, P, Ns, RADIX>() ? d0 + lds_adv<R, P>(r * Ns) : lds_at<R, P>(dd + r * Ns, c)] =
+                    v[it][r];
         }
     }
     __syncthreads();
@@ -231,10 +253,10 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, const cplx<R
         constexpr int RADIX = REM2 > 0 ? (1 << L2) : 3;
         constexpr int R2 = REM2 > 0 ? REM2 - L2 : 0, R3 = REM2 > 0 ? REM3 : REM3 - 1;
         if constexpr (IP) {
-            stockham_pass_ip<R, NC, P, INV, TWS, RADIX>(a, NS, tw);
+            stockham_pass_ip<R, NC, P, INV, TWS, RADIX, NS>(a, tw);
             return lds_fft<R, NC, P, INV, TWS, ML2, IP, NS * RADIX, R2, R3>(a, b, tw);
         } else {
-            stockham_pass<R, NC, P, INV, TWS, RADIX>(a, b, NS, tw);
+            stockham_pass<R, NC, P, INV, TWS, RADIX, NS>(a, b, tw);
             return lds_fft<R, NC, P, INV, TWS, ML2, IP, NS * RADIX, R2, R3>(b, a, tw);
         }
     }
