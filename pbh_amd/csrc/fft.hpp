@@ -133,15 +133,13 @@ template <typename R>
 constexpr int lds_pad(int P) { return sizeof(R) == 8 ? P : P + 1; }
 
 // Element (row, pencil c) of a P-pencil LDS tile, and the elements a tile of `rows` rows takes.
-// FP32 with P = 8: rows of 8 elements (16 dwords) in groups of 8 rows, each group followed by 8
-// unused elements (64 dwords; a group spans 144 dwords).  A wave's accesses in the pencil passes
-// are rows j .. j+7 (the radix-4 Stockham reads j + r*M and writes (j/Ns)*Ns*4 + r*Ns + j%Ns); with
-// 128 four-byte banks served by a 64-lane b64 access at once (the model that fits the measured
-// conflicts of every tile here), the 8 rows then fall on disjoint banks in every pass but the write
-// with Ns = 4 (two runs of 4 rows 32 dwords apart modulo 128).  Measured at the 256^3 crossing:
-// conflict cycles 1.62 -> 0.33 of the active LDS cycles against a stride of P + 2.  The padding is
-// additive, so the accesses of one thread keep constant distances (LDS instruction pairing).
-// Other tiles keep lds_pad.
+// FP32 with P = 8: rows of 8 elements (16 dwords) in groups of 8 rows, each group (128 dwords)
+// followed by 8 unused elements (16 dwords).  A wave's accesses in the pencil passes are rows
+// j .. j+7 (the radix-4 Stockham reads j + r*M and writes (j/Ns)*Ns*4 + r*Ns + j%Ns), which then
+// start on distinct 16-dword bank offsets in every pass but the write with Ns = 4 (two runs of 4
+// rows, 288 dwords apart).  Measured at the 256^3 crossing: conflict cycles 1.62 -> 0.33 of the
+// active LDS cycles against a stride of P + 2.  The padding is additive, so the accesses of one
+// thread keep constant distances (LDS instruction pairing).  Other tiles keep lds_pad.
 template <typename R, int P>
 constexpr bool lds_grouped() { return sizeof(R) == 4 && P == 8; }
 template <typename R, int P>
