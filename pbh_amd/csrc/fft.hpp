@@ -14,8 +14,8 @@
 // assembled only on the host side of the test entry points.
 //
 // LDS layout of a pencil block: element j of pencil c at buf[lds_at(j, c)], rows of P elements
-// (j*PP + c with PP = lds_pad(P): P for FP64, P + 1 for FP32, or rows of 8 in padded groups of 8
-// for FP32 with P = 8);
+// (j*PP + c with PP = lds_pad(P): P for rows of 128 bytes or more, else P + 1; FP32 with P = 8
+// in rows of 8 in padded groups of 8);
 // threads map pencil-fastest in the butterflies, so every LDS access of a wave is a run of
 // contiguous pencils.
 #pragma once
@@ -123,14 +123,16 @@ __device__ __forceinline__ void dft_reg(cplx<R>* v) {
 }
 
 // Row stride of a P-pencil LDS tile buf[j*PP + c].  Threads take consecutive pencils, so a wave
-// reads 64/P rows of P elements.  FP64 complex (16-byte elements, ds_read/write_b128): unpadded
-// rows; measured on MI355X at the 128^3 crossing, the x pass's bank-conflict cycles were 0.6 of its
-// active LDS cycles with P + 1 (1.1 with P + 2, 0.6 with P + 4) and 0 with P (P = 16; 0.02 with
-// P = 8), x_poisson 19.7 -> 18.9 us.  FP32 complex: P + 1 (with P = 8 the grouped layout of lds_at
-// instead: a stride of 18 dwords started the 8 rows of a wave unevenly, about 2e7 bank-conflict
-// cycles per 256^3 x-pass launch, and 20 dwords still left conflicts of 1.6x the active LDS cycles).
+// reads 64/P rows of P elements.  Rows of 128 bytes or more (FP64 complex with P >= 8, FP32 complex
+// with P >= 16) are unpadded: measured on MI355X, the FP64 x pass at the 128^3 crossing lost 0.6 of
+// its active LDS cycles to bank conflicts with P + 1 (1.1 with P + 2, 0.6 with P + 4) and none with
+// P (P = 16; 0.02 with P = 8), x_poisson 19.7 -> 18.9 us; the FP32 x passes of the 32^3 .. 128^3
+// boxes (P = 16, 32) went from 0.25 - 0.48 to 0, their active LDS cycles -15 %.  Shorter rows keep
+// P + 1 (FP32 with P = 8 takes the grouped layout of lds_at instead: a stride of 18 dwords started
+// the 8 rows of a wave unevenly, about 2e7 bank-conflict cycles per 256^3 x-pass launch, and 20
+// dwords still left conflicts of 1.6x the active LDS cycles).
 template <typename R>
-constexpr int lds_pad(int P) { return sizeof(R) == 8 ? P : P + 1; }
+constexpr int lds_pad(int P) { return P * (int)sizeof(cplx<R>) >= 128 ? P : P + 1; }
 
 // Element (row, pencil c) of a P-pencil LDS tile, and the elements a tile of `rows` rows takes.
 // FP32 with P = 8: rows of 8 elements (16 dwords) in groups of 8 rows, each group (128 dwords)
