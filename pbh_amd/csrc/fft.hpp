@@ -37,11 +37,14 @@ enum FftPassKind { PK_R2C = 0, PK_YFWD, PK_X, PK_YINV, PK_C2R };
 // radix 4 -> radix 8 (FP64) or 16 (FP32)): FP64 x 25.9 -> 24.9, y inv 24.2 -> 23.7, the other
 // passes equal within 1 %; FP32 r2c 56.9 -> 51.9, y fwd 45.0 -> 44.2, y inv 122 -> 114, c2r 171 ->
 // 148, but x 99 -> 116 (the Poisson x pass keeps its per-thread spectrum in registers, which radix
-// 16 crowds).  The radix fixes the rounding sequence, so it is a constant, not a tuning choice.
+// 16 crowds).  The FP32 x pass takes radix 8 (256 = 8 * 8 * 4: three LDS round trips instead of
+// four, and every write of the 8-pencil grouped tile on its own bank offsets, where radix 4's write
+// with Ns = 4 overlapped): mixed 256^3 x_poisson 80.4 -> 71 us.  The radix fixes the rounding
+// sequence, so it is a constant, not a tuning choice.
 template <typename R, int PK>
 constexpr int radix_l2() {
     if constexpr (sizeof(R) == 8) return (PK == PK_X || PK == PK_YINV) ? 3 : 2;
-    else return PK == PK_X ? 2 : 4;
+    else return PK == PK_X ? 3 : 4;
 }
 
 // Tile configuration V of a pass (every V computes the same bits, so pm3d.hip picks by timing):
