@@ -148,7 +148,7 @@ constexpr int lds_pad(int P) { return P * (int)sizeof(cplx<R>) >= 128 ? P : P + 
 template <typename R, int P>
 constexpr bool lds_grouped() { return sizeof(R) == 4 && P == 8; }
 template <typename R, int P>
-__device__ __forceinline__ int lds_at(int row, int c) {
+__host__ __device__ __forceinline__ constexpr int lds_at(int row, int c) {
     if constexpr (lds_grouped<R, P>()) return row * 8 + (row >> 3) * 8 + c;
     else return row * lds_pad<R>(P) + c;
 }
@@ -160,7 +160,7 @@ constexpr int lds_size(int rows) {
 // between cross d / 8 group ends: a constant for a constant d, which keeps the accesses of one
 // thread at immediate offsets of one address (the compiler cannot derive that from lds_at).
 template <typename R, int P>
-__device__ __forceinline__ constexpr int lds_adv(int d) {
+__host__ __device__ __forceinline__ constexpr int lds_adv(int d) {
     return lds_grouped<R, P>() ? d * 8 + (d >> 3) * 8 : d * lds_pad<R>(P);
 }
 // A radix-RADIX pass reads rows j + r*M and writes rows dd + r*NS with dd = (j/NS)*NS*RADIX + k,
