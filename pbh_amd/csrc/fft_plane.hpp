@@ -372,18 +372,28 @@ __device__ __forceinline__ void inv_plane_body(const cplx<R>* __restrict__ gspec
         }
     }
     __syncthreads();
-    // inverse y: RW-point transform of every kz column (pencil kz, element k')
-    tile_fft<R, RW, S, 1, RS, true, true, Q, IT_::ML2, NT>(T, tw);
+    // c2r uses only the real parts of the kz = 0 and kz = H columns' inverse y transforms, which are
+    // the transforms of their Hermitian parts (Herm(b)[k'] = (b[k'] + conj b[-k'])/2): column 0
+    // takes Herm(b_0) + i Herm(b_H), so one transform yields both (real and imaginary part) and the
+    // y pass runs over H columns, one per butterfly slot, instead of H + 1
+    for (int k = threadIdx.x; k <= RW / 2; k += NT) {
+        const int km = k == 0 ? 0 : RW - k;
+        const cplx<R> a = T[k], am = T[km], b = T[H * RS + k], bm = T[H * RS + km];
+        const R har = (a.re + am.re) * (R)0.5, hai = (a.im - am.im) * (R)0.5;
+        const R hbr = (b.re + bm.re) * (R)0.5, hbi = (b.im - bm.im) * (R)0.5;
+        T[k] = cplx<R>{har - hbi, hai + hbr};
+        if (km != k) T[km] = cplx<R>{har + hbi, hbr - hai};
+    }
+    __syncthreads();
+    // inverse y: RW-point transform of the kz columns 0 .. H-1 (pencil kz, element k')
+    tile_fft<R, RW, S - 1, 1, RS, true, true, Q, IT_::ML2, NT>(T, tw);
     // c2r input points of every row m (pencil m, element k): point k from bins k and H-k (bin 0
-    // pairs with the Nyquist bin; the imaginary parts of both are dropped)
+    // pairs with the Nyquist bin, whose real part column 0 carries as its imaginary part)
     for (int t = threadIdx.x; t < (H / 2 + 1) * RW; t += NT) {
         const int m = t % RW, k = t / RW;
         cplx<R> xk = T[k * RS + m];
-        cplx<R> xm = T[(k == 0 ? H : H - k) * RS + m];
-        if (k == 0) {
-            xk.im = (R)0;
-            xm.im = (R)0;
-        }
+        const cplx<R> xm = k == 0 ? cplx<R>{xk.im, (R)0} : T[(H - k) * RS + m];
+        if (k == 0) xk.im = (R)0;
         const cplx<R> pk = c2r_point(xk, xm, tw[k]);
         if (k != 0 && k != H - k) T[(H - k) * RS + m] = c2r_point(xm, xk, tw[H - k]);
         T[k * RS + m] = pk;
